@@ -1,0 +1,54 @@
+// kernels.h -- launch tables shared by kernels.hip and runtime.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "rig.h"
+
+namespace sister {
+
+enum Stage {
+    kStageCensus = 0,
+    kStageWta,
+    kStageMedianPrep,
+    kStageMask,
+    kStageMsum,
+    kStageRows,
+    kStageCols,
+    kStageDiag,   // diagonal and anti-diagonal sweeps share one table
+    kNumStages
+};
+
+// Everything one launch of a stage needs for a set of rigs (device pointers).
+struct LaunchSet {
+    const RigDesc* rigs;
+    int nrig;
+    const int* bs[kNumStages];      // block-start tables, nrig+1 entries each
+    int blocks[kNumStages];         // total blocks per stage
+    uint64_t* const* scratch;       // per rig: 8 median order-statistic maps
+    int maxD;
+    int maxCols;                    // max frame width over the rigs (median LDS)
+};
+
+// individual kernels, for per-kernel event timing
+enum KernelId {
+    kKCensus = 0, kKWta, kKMedianPrep, kKMedianSerial, kKMask, kKMsum,
+    kKRows, kKCols, kKDiag, kKAnti, kNumKernels
+};
+extern const char* const kKernelNames[kNumKernels];
+
+// optional hook called around every kernel launch
+struct KernelTimer {
+    virtual void begin(int kernel, hipStream_t st) = 0;
+    virtual void end(int kernel, hipStream_t st) = 0;
+    virtual ~KernelTimer() {}
+};
+
+void stage_blocks(const RigDesc& g, int out[kNumStages]);
+void launch_census(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
+void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
+void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
+void launch_mask(const LaunchSet& ls, int fmask, hipStream_t st, KernelTimer* t);
+void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t);
+// keep_sum: also store the final S of cells inside the crop (debug)
+void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, KernelTimer* t);
+
+}  // namespace sister

@@ -1,0 +1,832 @@
+// runtime.hip -- context, device memory, launch plans and the C ABI of
+// include/sister.h.
+//
+// The reference computes one rig per call on the host (compute_disparities,
+// SisterMultiviewDisparities.hpp:26-119).  Here a call takes any number of
+// rigs: the host builds one plan (rig table + per-stage block-start tables),
+// uploads it, and issues one launch per stage for all rigs together.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/sister.h"
+#include "kernels.h"
+#include "rig.h"
+
+using namespace sister;
+
+namespace {
+
+constexpr size_t kDefaultBudget = (size_t)64 << 30;
+constexpr size_t kAlign = 256;
+
+inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+// workspace layout of one rig (offsets from the workspace base)
+struct WsLayout {
+    size_t code, wta, med, mask, msum, sum, scratch, total;
+};
+
+WsLayout ws_layout(int W, int H, int D)
+{
+    const size_t Wp = (size_t)W + 2 * D, Hp = (size_t)H + 2 * D;
+    const size_t npx = Wp * Hp, ncell = npx * D;
+    WsLayout l;
+    size_t o = 0;
+    l.code = o;    o += align_up(8 * npx * sizeof(uint64_t));
+    l.wta = o;     o += align_up(8 * npx * sizeof(uint16_t));
+    l.med = o;     o += align_up(8 * npx * sizeof(uint16_t));
+    l.mask = o;    o += align_up(4 * npx);
+    l.msum = o;    o += align_up(ncell);
+    l.sum = o;     o += align_up(ncell * sizeof(uint16_t));
+    l.scratch = o; o += align_up(8 * npx * sizeof(uint64_t));
+    l.total = o;
+    return l;
+}
+
+// host-visible staging of one rig: 5 views, 4 uint16 maps, 1 float map
+size_t io_bytes(size_t WH)
+{
+    return 5 * align_up(WH) + 4 * align_up(WH * sizeof(uint16_t)) + align_up(WH * sizeof(float));
+}
+
+bool valid_dims(int W, int H, int D)
+{
+    return W >= 4 && H >= 4 && W % 4 == 0 && H % 4 == 0 && D >= 8 && D % 8 == 0 && D <= kMaxDisp;
+}
+
+struct Block {
+    char* ptr;
+    size_t bytes;
+};
+
+// Simple caching allocator bounded by the HBM budget.
+struct Pool {
+    size_t budget = kDefaultBudget;
+    size_t in_use = 0, held = 0, peak = 0;
+    std::multimap<size_t, char*> idle;
+
+    hipError_t get(size_t bytes, Block& out)
+    {
+        bytes = align_up(bytes);
+        auto it = idle.lower_bound(bytes);
+        if (it != idle.end() && it->first <= bytes * 2) {
+            out = {it->second, it->first};
+            idle.erase(it);
+            in_use += out.bytes;
+            peak = std::max(peak, in_use);
+            return hipSuccess;
+        }
+        if (held + bytes > budget) trim();
+        if (held + bytes > budget) return hipErrorOutOfMemory;
+        char* p = nullptr;
+        hipError_t e = hipMalloc((void**)&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            e = hipMalloc((void**)&p, bytes);
+            if (e != hipSuccess) { (void)hipGetLastError(); return e; }
+        }
+        held += bytes;
+        in_use += bytes;
+        peak = std::max(peak, in_use);
+        out = {p, bytes};
+        return hipSuccess;
+    }
+    void put(const Block& b)
+    {
+        if (!b.ptr) return;
+        in_use -= b.bytes;
+        idle.emplace(b.bytes, b.ptr);
+    }
+    void trim()
+    {
+        for (auto& kv : idle) { (void)hipFree(kv.second); held -= kv.first; }
+        idle.clear();
+    }
+    void release_all()
+    {
+        trim();
+    }
+};
+
+// HIP events around every kernel, accumulated per kernel id.
+struct EventTimer : KernelTimer {
+    struct Pair { hipEvent_t a, b; int id; };
+    std::vector<Pair> pending;
+    std::vector<hipEvent_t> free_events;
+    double ms[kNumKernels] = {};
+    int count[kNumKernels] = {};
+    hipEvent_t cur_begin = nullptr;
+
+    hipEvent_t ev()
+    {
+        if (!free_events.empty()) { hipEvent_t e = free_events.back(); free_events.pop_back(); return e; }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void begin(int, hipStream_t st) override
+    {
+        cur_begin = ev();
+        (void)hipEventRecord(cur_begin, st);
+    }
+    void end(int id, hipStream_t st) override
+    {
+        hipEvent_t e = ev();
+        (void)hipEventRecord(e, st);
+        pending.push_back({cur_begin, e, id});
+    }
+    void collect()
+    {
+        for (auto& p : pending) {
+            float t = 0.f;
+            if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+                ms[p.id] += t;
+                count[p.id] += 1;
+            }
+            free_events.push_back(p.a);
+            free_events.push_back(p.b);
+        }
+        pending.clear();
+    }
+    ~EventTimer() override
+    {
+        for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+        for (auto e : free_events) (void)hipEventDestroy(e);
+    }
+};
+
+// A prepared launch: rig table and block-start tables in device memory.
+struct Plan {
+    std::vector<RigDesc> host_rigs;
+    std::vector<Block> workspaces;
+    Block table{nullptr, 0};
+    LaunchSet ls{};
+};
+
+}  // namespace
+
+struct sister_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    Pool pool;
+    EventTimer timer;
+    bool profiling = false;
+    bool keep_sum = false;
+    Plan last;                 // plan of the last sister_run_device (kept for debug fetch)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    Plan graph_plan;
+    std::mutex mu;             // guards pool / plans between the serving thread and callers
+
+    // serving
+    struct Request {
+        uint64_t ticket;
+        sister_rig rig;
+        int modes;
+        std::chrono::steady_clock::time_point t_submit;
+        size_t footprint;
+    };
+    struct Inflight {
+        std::vector<Request> reqs;
+        Plan plan;
+        std::vector<Block> io;
+        hipStream_t stream;
+        hipEvent_t done;
+    };
+    std::thread server;
+    std::mutex smu;
+    std::condition_variable scv;
+    std::deque<Request> queue;
+    std::unordered_map<uint64_t, int> status;   // 0 queued, 1 running, 2 done, 3 failed
+    bool serving = false, stop = false;
+    int max_inflight = 4;
+    uint64_t next_ticket = 1;
+    std::vector<double> lat_ms;
+    uint64_t n_sub = 0, n_done = 0, n_rej = 0;
+    std::chrono::steady_clock::time_point t_first{}, t_last{};
+    bool have_first = false;
+};
+
+static int fail(sister_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                        \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            return fail(ctx, SISTER_EDEVICE, std::string(#expr ": ") + hipGetErrorString(_e));   \
+    } while (0)
+
+// Build the device plan for n rigs whose views/outputs are device pointers.
+static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
+{
+    p.host_rigs.clear();
+    p.workspaces.clear();
+    int maxD = 8, maxCols = 8;
+    for (int i = 0; i < n; i++) {
+        const sister_rig& r = rigs[i];
+        if (!valid_dims(r.width, r.height, r.disp_count))
+            return fail(c, SISTER_EINVAL, "rig " + std::to_string(i) + ": need W,H multiples of 4 and D a multiple of 8 in [8,512]");
+        for (int v = 0; v < 5; v++)
+            if (!r.view[v]) return fail(c, SISTER_EINVAL, "rig " + std::to_string(i) + ": null view");
+        if (r.stride < r.width) return fail(c, SISTER_EINVAL, "rig " + std::to_string(i) + ": stride < width");
+        RigDesc g{};
+        g.W = r.width; g.H = r.height; g.D = r.disp_count;
+        g.Wp = g.W + 2 * g.D; g.Hp = g.H + 2 * g.D;
+        g.stride = r.stride;
+        for (int v = 0; v < 5; v++) g.view[v] = r.view[v];
+        WsLayout l = ws_layout(g.W, g.H, g.D);
+        Block b;
+        if (c->pool.get(l.total, b) != hipSuccess) {
+            for (auto& w : p.workspaces) c->pool.put(w);
+            p.workspaces.clear();
+            return fail(c, SISTER_ENOMEM, "workspace of " + std::to_string(l.total) + " bytes exceeds the HBM budget");
+        }
+        p.workspaces.push_back(b);
+        const size_t npx = (size_t)g.Wp * g.Hp;
+        for (int k = 0; k < 8; k++) {
+            g.code[k] = (uint64_t*)(b.ptr + l.code) + k * npx;
+            g.wta[k] = (uint16_t*)(b.ptr + l.wta) + k * npx;
+            g.med[k] = (uint16_t*)(b.ptr + l.med) + k * npx;
+        }
+        for (int k = 0; k < 4; k++) g.mask[k] = (uint8_t*)(b.ptr + l.mask) + k * npx;
+        g.msum = (uint8_t*)(b.ptr + l.msum);
+        g.sum = (uint16_t*)(b.ptr + l.sum);
+        g.out[0] = r.out_multiview;
+        g.out[1] = r.out_horizontal;
+        g.out[2] = r.out_vertical;
+        g.disp = r.out_disp;
+        g.subpix = r.out_subpix;
+        maxD = std::max(maxD, g.D);
+        maxCols = std::max(maxCols, std::max(g.Wp, g.Hp));
+        p.host_rigs.push_back(g);
+    }
+    // table: rigs | scratch ptrs | bs[kNumStages][n+1]
+    const size_t rig_bytes = align_up(sizeof(RigDesc) * n);
+    const size_t scr_bytes = align_up(sizeof(uint64_t*) * n);
+    const size_t bs_bytes = align_up(sizeof(int) * (n + 1) * kNumStages);
+    std::vector<char> host(rig_bytes + scr_bytes + bs_bytes, 0);
+    std::memcpy(host.data(), p.host_rigs.data(), sizeof(RigDesc) * n);
+    uint64_t** scr = (uint64_t**)(host.data() + rig_bytes);
+    int* bs = (int*)(host.data() + rig_bytes + scr_bytes);
+    for (int i = 0; i < n; i++) {
+        WsLayout l = ws_layout(p.host_rigs[i].W, p.host_rigs[i].H, p.host_rigs[i].D);
+        scr[i] = (uint64_t*)(p.workspaces[i].ptr + l.scratch);
+    }
+    for (int s = 0; s < kNumStages; s++) p.ls.blocks[s] = 0;
+    for (int i = 0; i < n; i++) {
+        int nb[kNumStages];
+        stage_blocks(p.host_rigs[i], nb);
+        for (int s = 0; s < kNumStages; s++) {
+            bs[s * (n + 1) + i] = p.ls.blocks[s];
+            p.ls.blocks[s] += nb[s];
+        }
+    }
+    for (int s = 0; s < kNumStages; s++) bs[s * (n + 1) + n] = p.ls.blocks[s];
+    if (c->pool.get(host.size(), p.table) != hipSuccess) {
+        for (auto& w : p.workspaces) c->pool.put(w);
+        p.workspaces.clear();
+        return fail(c, SISTER_ENOMEM, "plan table allocation failed");
+    }
+    HIPCHK(c, hipMemcpy(p.table.ptr, host.data(), host.size(), hipMemcpyHostToDevice));
+    p.ls.rigs = (const RigDesc*)p.table.ptr;
+    p.ls.nrig = n;
+    p.ls.scratch = (uint64_t* const*)(p.table.ptr + rig_bytes);
+    for (int s = 0; s < kNumStages; s++)
+        p.ls.bs[s] = (const int*)(p.table.ptr + rig_bytes + scr_bytes) + s * (n + 1);
+    p.ls.maxD = maxD;
+    p.ls.maxCols = maxCols;
+    return SISTER_OK;
+}
+
+static void free_plan(sister_ctx* c, Plan& p)
+{
+    for (auto& w : p.workspaces) c->pool.put(w);
+    p.workspaces.clear();
+    c->pool.put(p.table);
+    p.table = {nullptr, 0};
+    p.host_rigs.clear();
+}
+
+// enqueue every stage of a plan on `st`
+static void enqueue(sister_ctx* c, const Plan& p, int modes, hipStream_t st, KernelTimer* t)
+{
+    const LaunchSet& ls = p.ls;
+    launch_census(ls, st, t);
+    launch_wta(ls, st, t);
+    launch_median(ls, st, t);
+    int fmask = 0;
+    if (modes & SISTER_MODE_MULTIVIEW) fmask |= 0xF;
+    if (modes & SISTER_MODE_HORIZONTAL) fmask |= 0x5;
+    if (modes & SISTER_MODE_VERTICAL) fmask |= 0xA;
+    launch_mask(ls, fmask, st, t);
+    static const int fuse[3] = {0xF, 0x5, 0xA};   // hpp:264-275: frames summed per mode
+    for (int m = 0; m < 3; m++) {
+        if (!(modes & (1 << m))) continue;
+        launch_msum(ls, fuse[m], st, t);
+        launch_sgm(ls, m, c->keep_sum ? 1 : 0, st, t);
+    }
+}
+
+static int check_modes(sister_ctx* c, const sister_rig* rigs, int n, int modes)
+{
+    if (modes <= 0 || modes > 7) return fail(c, SISTER_EINVAL, "modes must be a non-empty subset of SISTER_MODE_ALL");
+    for (int i = 0; i < n; i++) {
+        if ((modes & 1) && !rigs[i].out_multiview) return fail(c, SISTER_EINVAL, "out_multiview is NULL");
+        if ((modes & 2) && !rigs[i].out_horizontal) return fail(c, SISTER_EINVAL, "out_horizontal is NULL");
+        if ((modes & 4) && !rigs[i].out_vertical) return fail(c, SISTER_EINVAL, "out_vertical is NULL");
+    }
+    return SISTER_OK;
+}
+
+extern "C" {
+
+int sister_create(const sister_config* cfg, sister_ctx** out)
+{
+    if (!out) return SISTER_EINVAL;
+    *out = nullptr;
+    sister_ctx* c = new sister_ctx();
+    c->device = cfg ? cfg->device : 0;
+    if (cfg && cfg->hbm_budget) c->pool.budget = cfg->hbm_budget;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
+        (void)hipGetLastError();
+        delete c;
+        return SISTER_EDEVICE;
+    }
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SISTER_EDEVICE;
+    }
+    *out = c;
+    return SISTER_OK;
+}
+
+void sister_destroy(sister_ctx* c)
+{
+    if (!c) return;
+    sister_serve_stop(c);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    free_plan(c, c->last);
+    free_plan(c, c->graph_plan);
+    c->timer.collect();
+    c->pool.release_all();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* sister_last_error(const sister_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* sister_stream(sister_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+size_t sister_workspace_bytes(int W, int H, int D)
+{
+    if (!valid_dims(W, H, D)) return 0;
+    return ws_layout(W, H, D).total;
+}
+
+int sister_outside_reference_domain(int D) { return D > 256 ? 1 : 0; }
+
+int sister_run_device(sister_ctx* c, const sister_rig* rigs, int n, int modes)
+{
+    if (!c || !rigs || n <= 0) return fail(c, SISTER_EINVAL, "bad arguments");
+    int rc = check_modes(c, rigs, n, modes);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // the previous plan may still be in use
+    free_plan(c, c->last);
+    rc = build_plan(c, rigs, n, c->last);
+    if (rc) return rc;
+    enqueue(c, c->last, modes, c->stream, c->profiling ? &c->timer : nullptr);
+    HIPCHK(c, hipGetLastError());
+    return SISTER_OK;
+}
+
+int sister_sync(sister_ctx* c)
+{
+    if (!c) return SISTER_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->profiling) c->timer.collect();
+    return SISTER_OK;
+}
+
+int sister_compute_rig(sister_ctx* c, const sister_rig* r, int modes)
+{
+    if (!c || !r) return fail(c, SISTER_EINVAL, "bad arguments");
+    int rc = check_modes(c, r, 1, modes);
+    if (rc) return rc;
+    if (!valid_dims(r->width, r->height, r->disp_count))
+        return fail(c, SISTER_EINVAL, "need W,H multiples of 4 and D a multiple of 8 in [8,512]");
+    if (r->stride < r->width) return fail(c, SISTER_EINVAL, "stride < width");
+    const size_t WH = (size_t)r->width * r->height;
+    Block io;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        HIPCHK(c, hipSetDevice(c->device));
+        if (c->pool.get(io_bytes(WH), io) != hipSuccess)
+            return fail(c, SISTER_ENOMEM, "staging allocation failed");
+    }
+    char* p = io.ptr;
+    sister_rig d = *r;
+    d.stride = r->width;
+    for (int v = 0; v < 5; v++) {
+        if (!r->view[v]) { c->pool.put(io); return fail(c, SISTER_EINVAL, "null view"); }
+        hipError_t e = hipMemcpy2DAsync(p, r->width, r->view[v], r->stride, r->width, r->height, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) { c->pool.put(io); return fail(c, SISTER_EDEVICE, hipGetErrorString(e)); }
+        d.view[v] = (const uint8_t*)p;
+        p += align_up(WH);
+    }
+    uint16_t* dm = (uint16_t*)p; p += align_up(WH * 2);
+    uint16_t* dh = (uint16_t*)p; p += align_up(WH * 2);
+    uint16_t* dv = (uint16_t*)p; p += align_up(WH * 2);
+    uint16_t* dd = (uint16_t*)p; p += align_up(WH * 2);
+    float* dsp = (float*)p;
+    d.out_multiview = (modes & 1) ? dm : nullptr;
+    d.out_horizontal = (modes & 2) ? dh : nullptr;
+    d.out_vertical = (modes & 4) ? dv : nullptr;
+    d.out_disp = r->out_disp ? dd : nullptr;
+    d.out_subpix = r->out_subpix ? dsp : nullptr;
+    rc = sister_run_device(c, &d, 1, modes);
+    if (rc == SISTER_OK) {
+        hipError_t e = hipSuccess;
+        if (modes & 1) e = hipMemcpyAsync(r->out_multiview, dm, WH * 2, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && (modes & 2)) e = hipMemcpyAsync(r->out_horizontal, dh, WH * 2, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && (modes & 4)) e = hipMemcpyAsync(r->out_vertical, dv, WH * 2, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && r->out_disp) e = hipMemcpyAsync(r->out_disp, dd, WH * 2, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && r->out_subpix) e = hipMemcpyAsync(r->out_subpix, dsp, WH * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(c, SISTER_EDEVICE, hipGetErrorString(e));
+        if (c->profiling) c->timer.collect();
+    }
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->pool.put(io);
+    }
+    return rc;
+}
+
+int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes)
+{
+    if (!c || !rigs || n <= 0) return fail(c, SISTER_EINVAL, "bad arguments");
+    int rc = check_modes(c, rigs, n, modes);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    free_plan(c, c->graph_plan);
+    rc = build_plan(c, rigs, n, c->graph_plan);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue(c, c->graph_plan, modes, c->stream, nullptr);
+    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+    if (e != hipSuccess) return fail(c, SISTER_EDEVICE, std::string("capture: ") + hipGetErrorString(e));
+    HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    return SISTER_OK;
+}
+
+int sister_graph_launch(sister_ctx* c)
+{
+    if (!c || !c->graph_exec) return fail(c, SISTER_EINVAL, "no captured graph");
+    HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    return SISTER_OK;
+}
+
+int sister_profile_enable(sister_ctx* c, int on)
+{
+    if (!c) return SISTER_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timer.collect();
+    if (on) {
+        for (int i = 0; i < kNumKernels; i++) { c->timer.ms[i] = 0; c->timer.count[i] = 0; }
+    }
+    c->profiling = on != 0;
+    return SISTER_OK;
+}
+
+int sister_profile_read(sister_ctx* c, const char** names, double* ms, int* launches, int max)
+{
+    if (!c) return 0;
+    (void)hipStreamSynchronize(c->stream);
+    c->timer.collect();
+    int n = std::min(max, (int)kNumKernels);
+    for (int i = 0; i < n; i++) {
+        if (names) names[i] = kKernelNames[i];
+        if (ms) ms[i] = c->timer.ms[i];
+        if (launches) launches[i] = c->timer.count[i];
+    }
+    return n;
+}
+
+int sister_debug_fetch(sister_ctx* c, int index, const char* what, int which, void* dst, size_t bytes)
+{
+    if (!c || !what || !dst) return fail(c, SISTER_EINVAL, "bad arguments");
+    if (index < 0 || index >= (int)c->last.host_rigs.size()) return fail(c, SISTER_EINVAL, "no such rig in the last run");
+    const RigDesc& g = c->last.host_rigs[index];
+    const size_t npx = (size_t)g.Wp * g.Hp;
+    const void* src = nullptr;
+    size_t n = 0;
+    std::string w(what);
+    if (w == "census" && which >= 0 && which < 8) { src = g.code[which]; n = npx * 8; }
+    else if (w == "wta" && which >= 0 && which < 8) { src = g.wta[which]; n = npx * 2; }
+    else if (w == "med" && which >= 0 && which < 8) { src = g.med[which]; n = npx * 2; }
+    else if (w == "mask" && which >= 0 && which < 4) { src = g.mask[which]; n = npx; }
+    else if (w == "msum") { src = g.msum; n = npx * g.D; }
+    else if (w == "sum") { src = g.sum; n = npx * g.D * 2; }
+    else return fail(c, SISTER_EINVAL, "unknown intermediate " + w);
+    if (bytes < n) return fail(c, SISTER_EINVAL, "destination too small");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
+    return SISTER_OK;
+}
+
+// debug: keep S of crop cells (otherwise the final sweep consumes it)
+int sister_debug_keep_sum(sister_ctx* c, int on)
+{
+    if (!c) return SISTER_EINVAL;
+    c->keep_sum = on != 0;
+    return SISTER_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// serving
+// ---------------------------------------------------------------------------
+static size_t rig_footprint(const sister_rig& r)
+{
+    const size_t WH = (size_t)r.width * r.height;
+    return ws_layout(r.width, r.height, r.disp_count).total + io_bytes(WH) + 16 * kAlign;
+}
+
+static void serve_loop(sister_ctx* c)
+{
+    (void)hipSetDevice(c->device);
+    std::vector<sister_ctx::Inflight*> inflight;
+    std::vector<hipStream_t> streams;
+    for (int i = 0; i < c->max_inflight; i++) {
+        hipStream_t s;
+        (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        streams.push_back(s);
+    }
+    size_t reserved = 0;   // footprint of admitted requests
+    for (;;) {
+        // retire finished batches
+        for (size_t i = 0; i < inflight.size();) {
+            sister_ctx::Inflight* f = inflight[i];
+            hipError_t q = hipEventQuery(f->done);
+            if (q == hipErrorNotReady) { i++; continue; }
+            auto now = std::chrono::steady_clock::now();
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                free_plan(c, f->plan);
+                for (auto& b : f->io) c->pool.put(b);
+            }
+            {
+                std::lock_guard<std::mutex> lk(c->smu);
+                for (auto& r : f->reqs) {
+                    c->status[r.ticket] = (q == hipSuccess) ? 2 : 3;
+                    c->lat_ms.push_back(std::chrono::duration<double, std::milli>(now - r.t_submit).count());
+                    reserved -= r.footprint;
+                    c->n_done++;
+                }
+                c->t_last = now;
+            }
+            c->scv.notify_all();
+            (void)hipEventDestroy(f->done);
+            streams.push_back(f->stream);
+            delete f;
+            inflight.erase(inflight.begin() + i);
+        }
+        // admit: every queued request whose footprint fits, into one batch
+        std::vector<sister_ctx::Request> batch;
+        {
+            std::unique_lock<std::mutex> lk(c->smu);
+            if (c->stop && c->queue.empty() && inflight.empty()) break;
+            if (!streams.empty()) {
+                while (!c->queue.empty()) {
+                    auto& r = c->queue.front();
+                    if (reserved + r.footprint > c->pool.budget) {
+                        if (inflight.empty() && batch.empty()) {   // can never fit
+                            c->status[r.ticket] = 3;
+                            c->n_rej++;
+                            c->queue.pop_front();
+                            c->scv.notify_all();
+                            continue;
+                        }
+                        break;
+                    }
+                    reserved += r.footprint;
+                    c->status[r.ticket] = 1;
+                    batch.push_back(r);
+                    c->queue.pop_front();
+                }
+            }
+            if (batch.empty()) {
+                if (inflight.empty())
+                    c->scv.wait_for(lk, std::chrono::milliseconds(2));
+                else
+                    lk.unlock();
+            }
+        }
+        if (batch.empty()) {
+            if (!inflight.empty()) std::this_thread::sleep_for(std::chrono::microseconds(100));
+            continue;
+        }
+        // stage inputs, build one plan for the batch, launch on a free stream
+        auto* f = new sister_ctx::Inflight();
+        f->reqs = batch;
+        f->stream = streams.back();
+        streams.pop_back();
+        std::vector<sister_rig> dev(batch.size());
+        bool ok = true;
+        int modes = 0;
+        for (auto& r : batch) modes |= r.modes;
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            for (size_t i = 0; i < batch.size() && ok; i++) {
+                const sister_rig& h = batch[i].rig;
+                const size_t WH = (size_t)h.width * h.height;
+                Block b;
+                if (c->pool.get(io_bytes(WH), b) != hipSuccess) { ok = false; break; }
+                f->io.push_back(b);
+                char* p = b.ptr;
+                sister_rig d = h;
+                d.stride = h.width;
+                for (int v = 0; v < 5; v++) {
+                    (void)hipMemcpy2DAsync(p, h.width, h.view[v], h.stride, h.width, h.height, hipMemcpyHostToDevice, f->stream);
+                    d.view[v] = (const uint8_t*)p;
+                    p += align_up(WH);
+                }
+                d.out_multiview = (uint16_t*)p; p += align_up(WH * 2);
+                d.out_horizontal = (uint16_t*)p; p += align_up(WH * 2);
+                d.out_vertical = (uint16_t*)p; p += align_up(WH * 2);
+                d.out_disp = (uint16_t*)p; p += align_up(WH * 2);
+                d.out_subpix = (float*)p;
+                dev[i] = d;
+            }
+            if (ok) ok = build_plan(c, dev.data(), (int)dev.size(), f->plan) == SISTER_OK;
+            if (ok) enqueue(c, f->plan, modes, f->stream, nullptr);
+        }
+        for (size_t i = 0; ok && i < batch.size(); i++) {
+            const sister_rig& h = batch[i].rig;
+            const sister_rig& d = dev[i];
+            const size_t WH = (size_t)h.width * h.height;
+            if (h.out_multiview && (modes & 1)) (void)hipMemcpyAsync(h.out_multiview, d.out_multiview, WH * 2, hipMemcpyDeviceToHost, f->stream);
+            if (h.out_horizontal && (modes & 2)) (void)hipMemcpyAsync(h.out_horizontal, d.out_horizontal, WH * 2, hipMemcpyDeviceToHost, f->stream);
+            if (h.out_vertical && (modes & 4)) (void)hipMemcpyAsync(h.out_vertical, d.out_vertical, WH * 2, hipMemcpyDeviceToHost, f->stream);
+            if (h.out_disp && (modes & 1)) (void)hipMemcpyAsync(h.out_disp, d.out_disp, WH * 2, hipMemcpyDeviceToHost, f->stream);
+            if (h.out_subpix && (modes & 1)) (void)hipMemcpyAsync(h.out_subpix, d.out_subpix, WH * 4, hipMemcpyDeviceToHost, f->stream);
+        }
+        (void)hipEventCreateWithFlags(&f->done, hipEventDisableTiming);
+        (void)hipEventRecord(f->done, f->stream);
+        if (!ok) {
+            (void)hipStreamSynchronize(f->stream);
+            std::lock_guard<std::mutex> lk(c->mu);
+            free_plan(c, f->plan);
+            for (auto& b : f->io) c->pool.put(b);
+            std::lock_guard<std::mutex> lk2(c->smu);
+            for (auto& r : f->reqs) { c->status[r.ticket] = 3; reserved -= r.footprint; c->n_rej++; }
+            c->scv.notify_all();
+            (void)hipEventDestroy(f->done);
+            streams.push_back(f->stream);
+            delete f;
+            continue;
+        }
+        inflight.push_back(f);
+    }
+    for (auto s : streams) (void)hipStreamDestroy(s);
+}
+
+extern "C" {
+
+int sister_serve_start(sister_ctx* c, int max_inflight)
+{
+    if (!c) return SISTER_EINVAL;
+    std::lock_guard<std::mutex> lk(c->smu);
+    if (c->serving) return SISTER_OK;
+    c->max_inflight = std::max(1, std::min(max_inflight, 16));
+    c->stop = false;
+    c->serving = true;
+    c->lat_ms.clear();
+    c->n_sub = c->n_done = c->n_rej = 0;
+    c->have_first = false;
+    c->server = std::thread(serve_loop, c);
+    return SISTER_OK;
+}
+
+int sister_submit(sister_ctx* c, const sister_rig* r, int modes, uint64_t* ticket)
+{
+    if (!c || !r || !ticket) return fail(c, SISTER_EINVAL, "bad arguments");
+    if (!valid_dims(r->width, r->height, r->disp_count)) return fail(c, SISTER_EINVAL, "invalid rig shape or D");
+    if (modes <= 0 || modes > 7) return fail(c, SISTER_EINVAL, "bad modes");
+    for (int v = 0; v < 5; v++)
+        if (!r->view[v]) return fail(c, SISTER_EINVAL, "null view");
+    std::lock_guard<std::mutex> lk(c->smu);
+    if (!c->serving) return fail(c, SISTER_EINVAL, "serving not started");
+    sister_ctx::Request q;
+    q.ticket = c->next_ticket++;
+    q.rig = *r;
+    q.modes = modes;
+    q.t_submit = std::chrono::steady_clock::now();
+    q.footprint = rig_footprint(*r);
+    if (q.footprint > c->pool.budget) {
+        c->n_rej++;
+        return fail(c, SISTER_EBUSY, "rig footprint exceeds the HBM budget");
+    }
+    if (!c->have_first) { c->t_first = q.t_submit; c->have_first = true; }
+    c->status[q.ticket] = 0;
+    c->queue.push_back(q);
+    c->n_sub++;
+    *ticket = q.ticket;
+    c->scv.notify_all();
+    return SISTER_OK;
+}
+
+int sister_poll(sister_ctx* c, uint64_t t, int* done)
+{
+    if (!c || !done) return SISTER_EINVAL;
+    std::lock_guard<std::mutex> lk(c->smu);
+    auto it = c->status.find(t);
+    if (it == c->status.end()) return fail(c, SISTER_ENOTFOUND, "unknown ticket");
+    *done = it->second >= 2;
+    return it->second == 3 ? fail(c, SISTER_EDEVICE, "request failed") : SISTER_OK;
+}
+
+int sister_wait(sister_ctx* c, uint64_t t)
+{
+    if (!c) return SISTER_EINVAL;
+    std::unique_lock<std::mutex> lk(c->smu);
+    auto it = c->status.find(t);
+    if (it == c->status.end()) return fail(c, SISTER_ENOTFOUND, "unknown ticket");
+    c->scv.wait(lk, [&] { return c->status[t] >= 2; });
+    return c->status[t] == 3 ? fail(c, SISTER_EDEVICE, "request failed") : SISTER_OK;
+}
+
+int sister_serve_stop(sister_ctx* c)
+{
+    if (!c) return SISTER_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(c->smu);
+        if (!c->serving) return SISTER_OK;
+        c->stop = true;
+    }
+    c->scv.notify_all();
+    if (c->server.joinable()) c->server.join();
+    std::lock_guard<std::mutex> lk(c->smu);
+    c->serving = false;
+    return SISTER_OK;
+}
+
+int sister_stats_get(sister_ctx* c, sister_stats* s)
+{
+    if (!c || !s) return SISTER_EINVAL;
+    std::lock_guard<std::mutex> lk(c->smu);
+    std::memset(s, 0, sizeof(*s));
+    s->submitted = c->n_sub;
+    s->completed = c->n_done;
+    s->rejected = c->n_rej;
+    std::vector<double> v = c->lat_ms;
+    if (!v.empty()) {
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) { size_t i = (size_t)std::min<double>(v.size() - 1, std::floor(p * (v.size() - 1) + 0.5)); return v[i]; };
+        s->p50_ms = pct(0.50);
+        s->p99_ms = pct(0.99);
+        double sum = 0;
+        for (double x : v) sum += x;
+        s->mean_ms = sum / v.size();
+    }
+    if (c->have_first && c->n_done > 0) {
+        double span = std::chrono::duration<double>(c->t_last - c->t_first).count();
+        if (span > 0) s->rigs_per_s = c->n_done / span;
+    }
+    s->hbm_in_use = c->pool.in_use;
+    s->hbm_peak = c->pool.peak;
+    return SISTER_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,28 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    from oracle import oracle as O
+    if not os.path.exists(O.LIB_PATH):
+        O.build(ref=False)
+    return O
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx():
+    import sister_amd
+    ctx = sister_amd.Context()
+    yield ctx
+    ctx.close()
