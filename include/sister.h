@@ -53,7 +53,7 @@ typedef struct sister_ctx sister_ctx;
 typedef struct {
     int device;              /* HIP device ordinal */
     int flags;               /* reserved, 0 */
-    size_t hbm_budget;       /* device bytes the context may hold (0: 64 GiB) */
+    size_t hbm_budget;       /* device bytes the context may hold (0: 90% of free HBM) */
 } sister_config;
 
 /* One rig.  view[] = {center, right, top, left, bottom}, the constructor
@@ -123,7 +123,8 @@ int sister_profile_enable(sister_ctx* ctx, int on);
 int sister_profile_read(sister_ctx* ctx, const char** names, double* ms, int* launches, int max);
 /* Copy an intermediate of rig `index` of the last run to host memory:
    what = "census" (F*2+img, u64, frame coords), "wta" / "med" (F*2+side, u16),
-   "mask" (F, u8, padded H' x W'), "msum" (u8, H' x W' x D), "sum" (u16). */
+   "mask" (F, u8, padded H' x W'), "msum" (u8, H' x W' x Dp), "sum" (u16,
+   H' x W' x Dp, only with keep_sum); Dp = D rounded up to a multiple of 16. */
 int sister_debug_fetch(sister_ctx* ctx, int index, const char* what, int which, void* dst, size_t bytes);
 /* Keep the final aggregate S of every cell in the workspace (by default the
    last sweep consumes it for cells inside the crop). */

@@ -10,12 +10,21 @@ enum Stage {
     kStageWta,
     kStageMedianPrep,
     kStageMask,
-    kStageMsum,
-    kStageRows,
-    kStageCols,
-    kStageDiag,   // diagonal and anti-diagonal sweeps share one table
+    kStageMsum,     // cost sum, frames 0/180 (row runs)
+    kStageMsumV,    // cost sum, frames 90/270 (column runs)
     kNumStages
 };
+
+// SGM family launches run per group of rigs with the same (CH, KL): CH
+// chunks of 16 disparities per lane, KL lanes per chain.
+struct SgmGroup {
+    const RigDesc* rigs;     // device pointer to the group's first rig
+    int nrig;
+    const int* bs[3];        // block starts: rows, columns, diagonals (+anti)
+    int blocks[3];
+    int CH, KL;
+};
+constexpr int kMaxSgmGroups = 16;
 
 // Everything one launch of a stage needs for a set of rigs (device pointers).
 struct LaunchSet {
@@ -26,6 +35,8 @@ struct LaunchSet {
     uint64_t* const* scratch;       // per rig: 8 median order-statistic maps
     int maxD;
     int maxCols;                    // max frame width over the rigs (median LDS)
+    int ngroups;
+    SgmGroup groups[kMaxSgmGroups];
 };
 
 // individual kernels, for per-kernel event timing
@@ -42,7 +53,10 @@ struct KernelTimer {
     virtual ~KernelTimer() {}
 };
 
-void stage_blocks(const RigDesc& g, int out[kNumStages]);
+void stage_blocks(const RigDesc& g, int maxD, int out[kNumStages]);
+int sgm_pick_kl(long cols_chains);
+int sgm_pick_ch(int D, int KL);
+void sgm_group_blocks(const RigDesc& g, int KL, int out[3]);
 void launch_census(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t);

@@ -8,9 +8,9 @@
 //   doLRCheck + mask loops postprocess.cpp:318-341,     k_mask
 //                          hpp:201-251
 //   realign + weighted sum hpp:255-277                  k_msum
-//   accumulateCostsSSE     sgm.cpp:26-455               k_sgm_rows / k_sgm_cols /
-//                                                       k_sgm_diag / k_sgm_anti
-//   WTALeft on S, convert  hpp:283,111-118              fused into k_sgm_anti
+//   accumulateCostsSSE     sgm.cpp:26-455               k_fam<.., 0..3> (rows, columns,
+//                                                       diagonals, anti-diagonals)
+//   WTALeft on S, convert  hpp:283,111-118              fused into k_fam<.., 3>
 //
 // Every kernel takes the rig table plus a block-start table so one launch can
 // cover many rigs of different sizes: block b belongs to rig i when
@@ -348,319 +348,649 @@ __global__ __launch_bounds__(256) void k_mask(const RigDesc* __restrict__ rigs, 
 // K5 mask-weighted cost sum (hpp:255-277).  M(y,x,d) = sum over used frames F
 // with mask_F(y,x) = 1 of pop(CC_F[r][c] ^ CN_F[r][c-d]), (r,c) the frame
 // pixel of (y,x).  A set mask implies c >= D > d, so every term is a plain
-// popcount (<= 63) and M <= 252 fits a byte.  One wave per pixel, lanes over d.
+// popcount (<= 63) and M <= 252 fits a byte.
+// Each frame's partners of a pixel lie along its frame row, which is an
+// image row for frames 0/180 and an image column for frames 90/270, so two
+// kernels share the work: k_msum_rows (frames 0 and 180, one block = a run of
+// TX pixels of one row) writes M, k_msum_cols (frames 90 and 270, one block =
+// a run of TY pixels of one column) adds to it.  Each block stages the partner
+// window of its run (TX + D - 1 codes per frame) in LDS once; lanes sweep
+// (pixel, d) with consecutive lanes on consecutive d: conflict-free 8-byte
+// LDS reads and coalesced byte stores.  Cell stride Dp, zero past D.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_msum(const RigDesc* __restrict__ rigs, int nrig,
-                                              const int* __restrict__ bs, int fuse)
+constexpr int kRun = 64;    // pixels per block
+
+// side = 0: frames 0/180 along row y; side = 1: frames 90/270 along column x
+template <int SIDE>
+__global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ rigs, int nrig,
+                                                  const int* __restrict__ bs, int fuse)
 {
+    extern __shared__ uint64_t lds64[];
     const int ri = find_rig(bs, nrig, blockIdx.x);
     const RigDesc& g = rigs[ri];
-    const int lane = threadIdx.x & 63;
-    const long p = (long)(blockIdx.x - bs[ri]) * 4 + (threadIdx.x >> 6);
-    const long npx = (long)g.Wp * g.Hp;
-    if (p >= npx) return;
-    const int D = g.D;
-    const int y = (int)(p / g.Wp), x = (int)(p % g.Wp);
-    unsigned acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int F = 0; F < 4; F++) {
-        if (!(fuse & (1 << F)) || !g.mask[F][p]) continue;
-        int r, c;
-        orig_to_frame(g, F, y, x, r, c);
-        if (c < D) continue;   // a set mask implies c >= D; never index before the row
-        const int C = frame_cols(g, F);
-        const uint64_t cc = g.code[2 * F][(size_t)r * C + c];
-        const uint64_t* cn = g.code[2 * F + 1] + (size_t)r * C + c;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            int d = lane + 64 * j;
-            if (d < D) acc[j] += __popcll(cc ^ cn[-d]);
-        }
+    const int lb = blockIdx.x - bs[ri];
+    const int Wp = g.Wp, Hp = g.Hp, D = g.D, Dp = g.Dp;
+    const int FA = SIDE == 0 ? 0 : 1, FB = SIDE == 0 ? 2 : 3;   // frame pair
+    const bool useA = (fuse >> FA) & 1, useB = (fuse >> FB) & 1;
+    // the run: SIDE 0 -> row y, x in [p0, p0+kRun); SIDE 1 -> column x, y in [p0, p0+kRun)
+    const int runs = SIDE == 0 ? (Wp + kRun - 1) / kRun : (Hp + kRun - 1) / kRun;
+    const int line = lb / runs, p0 = (lb % runs) * kRun;
+    const int len = SIDE == 0 ? Wp : Hp;        // pixels along the line
+    const int SPAN = kRun + D - 1;
+    uint64_t* LA = lds64;                       // partners of frame FA
+    uint64_t* LB = LA + SPAN;                   // partners of frame FB
+    uint64_t* LCa = LB + SPAN;                  // centers
+    uint64_t* LCb = LCa + kRun;
+    uint8_t* LM = (uint8_t*)(LCb + kRun);
+    const int tid = threadIdx.x;
+    // frame row holding the line and the frame column of run pixel i:
+    //  SIDE 0: F0 row y col x;  F2 row y col W'-1-x
+    //  SIDE 1: F1 row W'-1-x col H'-1-y;  F3 row W'-1-x col y
+    const int frow = SIDE == 0 ? line : Wp - 1 - line;
+    const int fcols = SIDE == 0 ? Wp : Hp;
+    const uint64_t* NA = g.code[2 * FA + 1] + (size_t)frow * fcols;
+    const uint64_t* NB = g.code[2 * FB + 1] + (size_t)frow * fcols;
+    const uint64_t* CA = g.code[2 * FA] + (size_t)frow * fcols;
+    const uint64_t* CB = g.code[2 * FB] + (size_t)frow * fcols;
+    // frame A columns run with the line coordinate (x for F0, y for F3 ...)
+    // for SIDE 0: frame A = F0 (col = x), frame B = F2 (col = W'-1-x)
+    // for SIDE 1: frame A = F1 (col = H'-1-y), frame B = F3 (col = y)
+    // window of frame X: the frame columns [lo, lo + SPAN) covering every
+    // partner col - d of the run
+    int loA, loB;
+    if (SIDE == 0) { loA = p0 - (D - 1); loB = Wp - p0 - kRun - (D - 1); }
+    else { loA = Hp - p0 - kRun - (D - 1); loB = p0 - (D - 1); }
+    for (int k = tid; k < SPAN; k += 256) {
+        const int ca = loA + k, cb = loB + k;
+        LA[k] = (useA && ca >= 0 && ca < fcols) ? NA[ca] : 0;
+        LB[k] = (useB && cb >= 0 && cb < fcols) ? NB[cb] : 0;
     }
-    uint8_t* o = g.msum + p * D;
+    for (int i = tid; i < kRun; i += 256) {
+        const int pos = p0 + i;                 // x (SIDE 0) or y (SIDE 1)
+        uint64_t a = 0, b = 0;
+        uint8_t mb = 0;
+        if (pos < len) {
+            const int colA = SIDE == 0 ? pos : Hp - 1 - pos;
+            const int colB = SIDE == 0 ? Wp - 1 - pos : pos;
+            const int y = SIDE == 0 ? line : pos, x = SIDE == 0 ? pos : line;
+            const size_t q = (size_t)y * Wp + x;
+            if (useA) a = CA[colA];
+            if (useB) b = CB[colB];
+            // a set mask implies frame column >= D; the check keeps indices in range
+            if (useA && g.mask[FA][q] && colA >= D) mb |= 1;
+            if (useB && g.mask[FB][q] && colB >= D) mb |= 2;
+        }
+        LCa[i] = a;
+        LCb[i] = b;
+        LM[i] = mb;
+    }
+    __syncthreads();
+    const int QPP = Dp / 4;                     // lanes per pixel (d = q + j*QPP)
+    const int ppass = 256 / QPP;
+    const int slot = tid / QPP, q = tid % QPP;
+    if (slot >= ppass) return;
+    for (int base = 0; base < kRun; base += ppass) {
+        const int i = base + slot;
+        if (i >= kRun) break;
+        const int pos = p0 + i;
+        if (pos >= len) break;
+        const int y = SIDE == 0 ? line : pos, x = SIDE == 0 ? pos : line;
+        uint8_t* o = g.msum + ((size_t)y * Wp + x) * Dp;
+        unsigned acc[4] = {0, 0, 0, 0};
+        if (SIDE == 1 && (fuse & 5)) {
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        int d = lane + 64 * j;
-        if (d < D) o[d] = (uint8_t)acc[j];
+            for (int j = 0; j < 4; j++) acc[j] = o[q + j * QPP];
+        }
+        const uint8_t mb = LM[i];
+        // partner of run pixel i at disparity d: frame column col - d, whose
+        // window index is col - d - lo
+        if (mb & 1) {
+            const uint64_t cc = LCa[i];
+            const int colA = SIDE == 0 ? pos : Hp - 1 - pos;
+            const uint64_t* w = LA + (colA - loA);
+#pragma unroll
+            for (int j = 0; j < 4; j++) { const int d = q + j * QPP; if (d < D) acc[j] += __popcll(cc ^ w[-d]); }
+        }
+        if (mb & 2) {
+            const uint64_t cc = LCb[i];
+            const int colB = SIDE == 0 ? Wp - 1 - pos : pos;
+            const uint64_t* w = LB + (colB - loB);
+#pragma unroll
+            for (int j = 0; j < 4; j++) { const int d = q + j * QPP; if (d < D) acc[j] += __popcll(cc ^ w[-d]); }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[q + j * QPP] = (uint8_t)acc[j];
     }
 }
 
 // ---------------------------------------------------------------------------
-// K6 SGM (sgm.cpp:26-455).  S is the sum of eight path costs; each path is an
-// independent chain, so the four chain families (rows, columns, diagonals,
-// anti-diagonals) are swept by separate kernels, one wave per chain and both
-// directions per wave, lanes over d (d = lane + 64 j).
+// K6 SGM (sgm.cpp:26-455).  S is the sum of eight path costs; each path is a
+// family of independent chains (rows, columns, diagonals, anti-diagonals), each
+// swept in both directions.  One lane owns one chain and keeps all D
+// disparities of its path state in registers as packed u16 pairs, so d-1 / d+1
+// are a byte-permute of neighbouring registers, the minimum over d is a
+// running packed min, and no lane talks to another.
 //
 // Standard step (rows >= 1 of a pass, all paths; sgm.cpp:278-383):
 //   L(d) = M(d) + min(L'(d), min(L'(d-1), L'(d+1)) + P1, minL' + P2) - minL'
-// with L'(-1) = L'(D) = guard.  Path values stay <= M + P2 <= 352 and S <=
-// 8*352, so the reference's saturating u16 arithmetic never saturates and
-// plain u32 arithmetic gives the same numbers.
+// with L'(-1) = L'(D) = guard.  (L - M) is in [0, P2] and M <= 252, so the
+// reference's saturating u16 arithmetic never saturates and u16 lanes hold it.
 // Pass structure reproduced exactly (sgm.cpp:89-207):
-//   - first line of each pass (row 0 down, row H'-1 up): paths r1,r2,r3 only
-//     take the state c~ = (M == 255 ? 0 : M) and add nothing to S; the
-//     horizontal path r0 runs on c~ with its result truncated to 8 bits
-//     (uint8 saturate_cast) and is the only contribution on that row;
-//   - r0 restarts from L' = 0, minL' = 0 on every other row (L = M at the
-//     first pixel);
-//   - r1 / r3 entering from outside the left/right border see L' = guard and
-//     minL' = 0, giving L = M + P2.
+//   - on the first line of each pass (row 0 down, row H'-1 up) paths r1, r2,
+//     r3 only take the state c~ = (M == 255 ? 0 : M) and add nothing to S; the
+//     horizontal path r0 runs on c~ truncated to 8 bits (uint8 saturate_cast)
+//     and is the only contribution of that pass on that row;
+//   - r0 restarts every other row from L' = 0, minL' = 0 (L = M);
+//   - r1 / r3 entering from beyond the left/right border see L' = guard and
+//     minL' = 0, so L = M + P2 (the standard step from a guard state).
+// Each family stores, for rows 1..H'-2, (L_fwd - M) + (L_bwd - M) in [0, 200]
+// as a byte (part[f]), and for rows 0 and H'-1 its exact contribution as u16
+// (edge[f]).  The anti-diagonal family runs last: its upward sweep forms
+//   S = 8 M + sum_f part[f]      (rows 1..H'-2)
+// and takes the winner (hpp:283) for cells inside the crop.
 // ---------------------------------------------------------------------------
-template <int J>
-struct Vec { unsigned v[J]; };
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kGuard2 = 0x3FFF3FFFu;
 
-template <int J>
-__device__ __forceinline__ void sgm_step(const Vec<J>& Lp, unsigned mp, const Vec<J>& c, Vec<J>& Ln,
-                                         unsigned& mn, int D, int lane, bool trunc8)
+// compile-time loop: f(integral_constant<int, I>) for I in [B, E), so register
+// arrays indexed by the chunk number are never indexed at run time
+template <int I> struct IC { static constexpr int value = I; };
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f)
 {
-    unsigned lmin = 0xFFFFFFFFu;
-    const unsigned p2 = mp + kP2;
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-        unsigned fl = (j == 0) ? kGuard : (unsigned)__builtin_amdgcn_readlane((int)Lp.v[j > 0 ? j - 1 : 0], 63);
-        unsigned fr = (j == J - 1) ? kGuard : (unsigned)__builtin_amdgcn_readlane((int)Lp.v[j + 1 < J ? j + 1 : j], 0);
-        unsigned nb = min(shr1(Lp.v[j], fl), shl1(Lp.v[j], fr)) + kP1;
-        unsigned t = min(min(nb, Lp.v[j]), p2);
-        unsigned v = c.v[j] + t - mp;
-        if (trunc8) v &= 0xFFu;
-        v = (lane + 64 * j < D) ? v : kGuard;
-        Ln.v[j] = v;
-        lmin = min(lmin, v);
-    }
-    mn = wave_min(lmin);
-}
-
-template <int J>
-__device__ __forceinline__ void load_cost(Vec<J>& c, const uint8_t* cell, int D, int lane, bool first_line)
-{
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-        int d = lane + 64 * j;
-        unsigned v = (d < D) ? cell[d] : 0u;
-        if (first_line && v == 255u) v = 0u;   // paramInvalidDispCost (sgm.cpp:109-110,146-147)
-        c.v[j] = v;
+    if constexpr (B < E) {
+        f(IC<B>{});
+        static_for<B + 1, E>(f);
     }
 }
 
-template <int J>
-__device__ __forceinline__ void init_state(Vec<J>& L, unsigned& m, const Vec<J>& c, int D, int lane, unsigned add)
+__device__ __forceinline__ us2 U2(uint32_t v) { return __builtin_bit_cast(us2, v); }
+__device__ __forceinline__ uint32_t W2(us2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ us2 umin2(us2 a, us2 b) { return __builtin_elementwise_min(a, b); }
+// bytes b0,b1 of w -> pair (b0, b1); bytes b2,b3 -> pair (b2, b3)
+__device__ __forceinline__ uint32_t bytes01(uint32_t w) { return __builtin_amdgcn_perm(w, w, 0x0c010c00u); }
+__device__ __forceinline__ uint32_t bytes23(uint32_t w) { return __builtin_amdgcn_perm(w, w, 0x0c030c02u); }
+// low bytes of pairs a, b -> one word (a.lo, a.hi, b.lo, b.hi)
+__device__ __forceinline__ uint32_t pack4(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x06040200u); }
+__device__ __forceinline__ uint32_t dup16(uint32_t v) { return v | (v << 16); }
+// 255 -> 0 in both halves (paramInvalidDispCost, sgm.cpp:109-110,146-147)
+__device__ __forceinline__ uint32_t tilde2(uint32_t p)
 {
-    unsigned lmin = 0xFFFFFFFFu;
+    uint32_t lo = (p & 0xFFFFu) == 255u ? 0u : (p & 0xFFFFu);
+    uint32_t hi = (p >> 16) == 255u ? 0u : (p >> 16);
+    return lo | (hi << 16);
+}
+
+// unpack one 16-byte chunk of a u8 volume into 8 pairs, and back
+__device__ __forceinline__ void unpack16(const uint4& v, uint32_t (&p)[8])
+{
+    p[0] = bytes01(v.x); p[1] = bytes23(v.x);
+    p[2] = bytes01(v.y); p[3] = bytes23(v.y);
+    p[4] = bytes01(v.z); p[5] = bytes23(v.z);
+    p[6] = bytes01(v.w); p[7] = bytes23(v.w);
+}
+__device__ __forceinline__ uint4 pack16(const uint32_t (&p)[8])
+{
+    return make_uint4(pack4(p[0], p[1]), pack4(p[2], p[3]), pack4(p[4], p[5]), pack4(p[6], p[7]));
+}
+
+// DPP within a 16-lane row: lane i gets lane i-1 (shr) / i+1 (shl); `fill`
+// where the source is outside the row
+__device__ __forceinline__ uint32_t row_shr1(uint32_t v, uint32_t fill)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x111, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t row_shl1(uint32_t v, uint32_t fill)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x101, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_min(uint32_t v)
+{
+    return min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, CTRL, 0xF, 0xF, false));
+}
+// minimum over each aligned group of KL lanes (KL in 1,2,4,8,16), in every lane
+template <int KL>
+__device__ __forceinline__ uint32_t group_min(uint32_t v)
+{
+    if (KL >= 2) v = dpp_min<0xB1>(v);    // quad_perm [1,0,3,2]
+    if (KL >= 4) v = dpp_min<0x4E>(v);    // quad_perm [2,3,0,1]
+    if (KL >= 8) v = dpp_min<0x141>(v);   // row_half_mirror
+    if (KL >= 16) v = dpp_min<0x140>(v);  // row_mirror
+    return v;
+}
+
+// One chain is held by KL consecutive lanes; lane `lid` of the group owns
+// chunks [lid*CH, lid*CH + CH) of the chain's Dp/16 chunks (16 disparities,
+// 8 packed pairs each).  Chunks at or beyond nq are padding and stay guards.
+template <int CH>
+struct LaneState {
+    uint32_t L[8 * CH];
+    uint32_t mp;
+};
+
+// loads of one visited cell, issued a row ahead of their use
+template <int CH, int NO>
+struct RowBuf {
+    uint4 m[CH];      // cost M
+    uint4 p[CH];      // this family's forward (L - M), backward sweeps
+    uint4 o[NO * CH]; // the other families' parts (final sweep)
+};
+
+struct ChainGeo {
+    int lid;        // lane within the chain group
+    int nq;         // valid chunks of the chain
+    bool half_tail; // D % 16 == 8: the last valid chunk holds 4 real pairs
+};
+
+// one standard step over the lane's chunks.  m = cost pairs per chunk;
+// returns (L - M) pairs in tp.  trunc: 8-bit first-line truncation.
+template <int CH, int KL>
+__device__ __forceinline__ void lane_step(LaneState<CH>& st, const ChainGeo& cg, const uint32_t (&m)[CH][8],
+                                          uint32_t (&tp)[CH][8], bool trunc)
+{
+    const uint32_t nbl = row_shr1(st.L[8 * CH - 1], kGuard2);   // left lane's last pair (old)
+    const uint32_t nbr = row_shl1(st.L[0], kGuard2);            // right lane's first pair (old)
+    uint32_t prv = (cg.lid == 0) ? kGuard2 : nbl;
+    const uint32_t rfill = (cg.lid == KL - 1) ? kGuard2 : nbr;
+    const uint32_t mpp = dup16(st.mp);
+    const us2 p2t = U2(mpp) + (us2){(unsigned short)kP2, (unsigned short)kP2};
+    us2 rmin = U2(0xFFFFFFFFu);
+    static_for<0, CH>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int qg = cg.lid * CH + q;
+        const bool valid = qg < cg.nq;
+        const bool ht = cg.half_tail && qg == cg.nq - 1;
 #pragma unroll
-    for (int j = 0; j < J; j++) {
-        unsigned v = (lane + 64 * j < D) ? c.v[j] + add : kGuard;
-        L.v[j] = v;
-        lmin = min(lmin, v);
-    }
-    m = wave_min(lmin);
-}
-
-template <int J>
-__device__ __forceinline__ void load_sum(Vec<J>& s, const uint16_t* cell, int D, int lane)
-{
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-        int d = lane + 64 * j;
-        s.v[j] = (d < D) ? cell[d] : 0u;
-    }
-}
-
-template <int J>
-__device__ __forceinline__ void store_sum(uint16_t* cell, const Vec<J>& s, int D, int lane)
-{
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-        int d = lane + 64 * j;
-        if (d < D) cell[d] = (uint16_t)s.v[j];
-    }
-}
-
-template <int J>
-__device__ __forceinline__ void acc_sum(uint16_t* cell, const Vec<J>& L, int D, int lane)
-{
-    Vec<J> s;
-    load_sum<J>(s, cell, D, lane);
-#pragma unroll
-    for (int j = 0; j < J; j++) s.v[j] += L.v[j];
-    store_sum<J>(cell, s, D, lane);
-}
-
-// one chain walked cell by cell: visit(i) -> (y, x) of the i-th cell
-// kind: 0 = first line (state c~, no contribution), 1 = guard entry (M + P2,
-// contributes), 2 = restart of r0 (L = M, contributes), 3 = first line of r0
-// (c~, contributes, later steps truncated)
-template <int J, class Pos, class Emit>
-__device__ __forceinline__ void run_chain(const RigDesc& g, int n, int kind, Pos pos, Emit emit, int lane)
-{
-    const int D = g.D;
-    const size_t rowc = (size_t)g.Wp * D;
-    Vec<J> L, c;
-    unsigned m;
-    int y, x;
-    pos(0, y, x);
-    load_cost<J>(c, g.msum + y * rowc + (size_t)x * D, D, lane, kind == 0 || kind == 3);
-    init_state<J>(L, m, c, D, lane, kind == 1 ? (unsigned)kP2 : 0u);
-    if (kind != 0) emit(y, x, L);
-    const bool trunc = (kind == 3);   // the whole chain lies on the first line
-    for (int i = 1; i < n; i++) {
-        pos(i, y, x);
-        load_cost<J>(c, g.msum + y * rowc + (size_t)x * D, D, lane, trunc);
-        Vec<J> Ln;
-        unsigned mn;
-        sgm_step<J>(L, m, c, Ln, mn, D, lane, trunc);
-        L = Ln;
-        m = mn;
-        emit(y, x, L);
-    }
-}
-
-// r0 (horizontal): one wave per row; forward sweep writes S, backward adds.
-template <int J>
-__global__ __launch_bounds__(256) void k_sgm_rows(const RigDesc* __restrict__ rigs, int nrig,
-                                                  const int* __restrict__ bs)
-{
-    const int ri = find_rig(bs, nrig, blockIdx.x);
-    const RigDesc& g = rigs[ri];
-    const int lane = threadIdx.x & 63;
-    const int y = (blockIdx.x - bs[ri]) * 4 + (threadIdx.x >> 6);
-    if (y >= g.Hp) return;
-    const int D = g.D, Wp = g.Wp;
-    uint16_t* srow = g.sum + (size_t)y * Wp * D;
-    // pass 0 direction: left to right; row 0 is the pass-0 first line
-    run_chain<J>(g, Wp, y == 0 ? 3 : 2,
-        [&](int i, int& yy, int& xx) { yy = y; xx = i; },
-        [&](int yy, int xx, const Vec<J>& L) { store_sum<J>(srow + (size_t)xx * D, L, D, lane); }, lane);
-    // pass 1 direction: right to left; row H'-1 is the pass-1 first line
-    run_chain<J>(g, Wp, y == g.Hp - 1 ? 3 : 2,
-        [&](int i, int& yy, int& xx) { yy = y; xx = Wp - 1 - i; },
-        [&](int yy, int xx, const Vec<J>& L) { acc_sum<J>(srow + (size_t)xx * D, L, D, lane); }, lane);
-}
-
-// r2 (vertical): one wave per column
-template <int J>
-__global__ __launch_bounds__(256) void k_sgm_cols(const RigDesc* __restrict__ rigs, int nrig,
-                                                  const int* __restrict__ bs)
-{
-    const int ri = find_rig(bs, nrig, blockIdx.x);
-    const RigDesc& g = rigs[ri];
-    const int lane = threadIdx.x & 63;
-    const int x = (blockIdx.x - bs[ri]) * 4 + (threadIdx.x >> 6);
-    if (x >= g.Wp) return;
-    const int D = g.D, Hp = g.Hp;
-    const size_t rowc = (size_t)g.Wp * D;
-    auto emit = [&](int yy, int xx, const Vec<J>& L) { acc_sum<J>(g.sum + yy * rowc + (size_t)xx * D, L, D, lane); };
-    run_chain<J>(g, Hp, 0, [&](int i, int& yy, int& xx) { yy = i; xx = x; }, emit, lane);
-    run_chain<J>(g, Hp, 0, [&](int i, int& yy, int& xx) { yy = Hp - 1 - i; xx = x; }, emit, lane);
-}
-
-// r1 (down-right, pass 0) and its pass-1 mirror (up-left): chain k = x - y
-template <int J>
-__global__ __launch_bounds__(256) void k_sgm_diag(const RigDesc* __restrict__ rigs, int nrig,
-                                                  const int* __restrict__ bs)
-{
-    const int ri = find_rig(bs, nrig, blockIdx.x);
-    const RigDesc& g = rigs[ri];
-    const int lane = threadIdx.x & 63;
-    const int kk = (blockIdx.x - bs[ri]) * 4 + (threadIdx.x >> 6);
-    const int Wp = g.Wp, Hp = g.Hp, D = g.D;
-    if (kk >= Wp + Hp - 1) return;
-    const int k = kk - (Hp - 1);                 // x - y
-    const int y0 = max(0, -k);
-    const int y1 = min(Hp - 1, Wp - 1 - k);
-    const int n = y1 - y0 + 1;
-    const size_t rowc = (size_t)Wp * D;
-    auto emit = [&](int yy, int xx, const Vec<J>& L) { acc_sum<J>(g.sum + yy * rowc + (size_t)xx * D, L, D, lane); };
-    // down: enters at row 0 (first line) or at column 0 (left border guard)
-    run_chain<J>(g, n, y0 == 0 ? 0 : 1, [&](int i, int& yy, int& xx) { yy = y0 + i; xx = yy + k; }, emit, lane);
-    // up: enters at row H'-1 (first line) or at column W'-1 (right border guard)
-    run_chain<J>(g, n, y1 == Hp - 1 ? 0 : 1, [&](int i, int& yy, int& xx) { yy = y1 - i; xx = yy + k; }, emit, lane);
-}
-
-// r3 (down-left, pass 0) and its pass-1 mirror (up-right): chain s = x + y.
-// The upward sweep is the last contribution to every cell it visits, so it
-// also takes the final winner (hpp:283) for cells inside the crop and writes
-// the API outputs (hpp:111-118) plus the sub-pixel refinement.
-template <int J>
-__global__ __launch_bounds__(256) void k_sgm_anti(const RigDesc* __restrict__ rigs, int nrig,
-                                                  const int* __restrict__ bs, int mode, int keep_sum)
-{
-    const int ri = find_rig(bs, nrig, blockIdx.x);
-    const RigDesc& g = rigs[ri];
-    const int lane = threadIdx.x & 63;
-    const int s = (blockIdx.x - bs[ri]) * 4 + (threadIdx.x >> 6);
-    const int Wp = g.Wp, Hp = g.Hp, D = g.D;
-    if (s >= Wp + Hp - 1) return;
-    const int y0 = max(0, s - (Wp - 1));
-    const int y1 = min(Hp - 1, s);
-    const int n = y1 - y0 + 1;
-    const size_t rowc = (size_t)Wp * D;
-    auto emit = [&](int yy, int xx, const Vec<J>& L) { acc_sum<J>(g.sum + yy * rowc + (size_t)xx * D, L, D, lane); };
-    // down: from (y-1, x+1); enters at row 0 or at column W'-1
-    run_chain<J>(g, n, y0 == 0 ? 0 : 1, [&](int i, int& yy, int& xx) { yy = y0 + i; xx = s - yy; }, emit, lane);
-    // up: from (y+1, x-1); enters at row H'-1 or at column 0
-    uint16_t* outm = g.out[mode];
-    auto final_emit = [&](int yy, int xx, const Vec<J>& L) {
-        uint16_t* cell = g.sum + yy * rowc + (size_t)xx * D;
-        Vec<J> S;
-        load_sum<J>(S, cell, D, lane);
-#pragma unroll
-        for (int j = 0; j < J; j++) S.v[j] += L.v[j];
-        const int oy = yy - D, ox = xx - D;
-        const bool crop = !(oy < 0 || oy >= g.H || ox < 0 || ox >= g.W);
-        if (!crop || keep_sum) store_sum<J>(cell, S, D, lane);
-        if (!crop) return;
-        // WTALeft over the full range (x >= D > D-1), lowest index on ties
-        unsigned key = 0xFFFFFFFFu;
-#pragma unroll
-        for (int j = 0; j < J; j++) {
-            int d = lane + 64 * j;
-            if (d < D) key = min(key, (S.v[j] << 9) | (unsigned)d);
+        for (int i = 0; i < 8; i++) {
+            const int j = 8 * q + i;
+            const uint32_t cur = st.L[j];
+            const uint32_t nxt = (j + 1 < 8 * CH) ? st.L[j + 1] : rfill;
+            const us2 left = U2(__builtin_amdgcn_alignbit(cur, prv, 16));
+            const us2 right = U2(__builtin_amdgcn_alignbit(nxt, cur, 16));
+            const us2 nb = umin2(left, right) + (us2){(unsigned short)kP1, (unsigned short)kP1};
+            const us2 t = umin2(umin2(nb, U2(cur)), p2t);
+            const us2 d = t - U2(mpp);
+            us2 l = d + U2(m[q][i]);
+            if (trunc) l = l & (us2){0xFF, 0xFF};
+            const bool pad = !valid || (ht && i >= 4);
+            l = pad ? U2(kGuard2) : l;
+            tp[q][i] = W2(d);
+            st.L[j] = W2(l);
+            prv = cur;
+            rmin = umin2(rmin, l);
         }
-        key = wave_min(key);
-        const unsigned best = key & 511u;
-        const size_t o = (size_t)oy * g.W + ox;
-        if (lane == 0) {
-            if (outm) outm[o] = (uint16_t)min(best * 255u, 65535u);
-            if (mode == 0 && g.disp) g.disp[o] = (uint16_t)best;
-        }
-        if (mode == 0 && g.subpix) {
-            // parabola through S(d*-1), S(d*), S(d*+1) (build-defined refinement)
-            float delta = 0.0f;
-            if (best > 0 && (int)best < D - 1) {
-                unsigned sv[3];
+    });
+    st.mp = group_min<KL>(min((uint32_t)rmin.x, (uint32_t)rmin.y));
+}
+
+// state <- given pairs (first-line c~ or a restart), padding as guards
+template <int CH, int KL>
+__device__ __forceinline__ void lane_set(LaneState<CH>& st, const ChainGeo& cg, const uint32_t (&m)[CH][8], bool on)
+{
+    us2 rmin = U2(0xFFFFFFFFu);
+    static_for<0, CH>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int qg = cg.lid * CH + q;
+        const bool valid = on && qg < cg.nq;
+        const bool ht = cg.half_tail && qg == cg.nq - 1;
 #pragma unroll
-                for (int t = 0; t < 3; t++) {
-                    const unsigned dd = best - 1 + t;
-                    const unsigned jj = dd >> 6, ll = dd & 63;
-                    unsigned val = 0;
-#pragma unroll
-                    for (int j = 0; j < J; j++)
-                        if ((unsigned)j == jj) val = (unsigned)__builtin_amdgcn_readlane((int)S.v[j], (int)ll);
-                    sv[t] = val;
-                }
-                const int den = (int)sv[0] - 2 * (int)sv[1] + (int)sv[2];
-                if (den > 0) delta = (float)((int)sv[0] - (int)sv[2]) / (2.0f * (float)den);
-            }
-            if (lane == 0) g.subpix[o] = (float)best + delta;
+        for (int i = 0; i < 8; i++) {
+            const uint32_t l = (!valid || (ht && i >= 4)) ? kGuard2 : m[q][i];
+            st.L[8 * q + i] = l;
+            rmin = umin2(rmin, U2(l));
         }
+    });
+    st.mp = group_min<KL>(min((uint32_t)rmin.x, (uint32_t)rmin.y));
+    st.mp = st.mp == 0x3FFFu ? 0u : st.mp;   // a chain with no state yet: minL' = 0
+}
+
+template <int CH>
+__device__ __forceinline__ void lane_guard(LaneState<CH>& st, bool keep)
+{
+    static_for<0, 8 * CH>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        st.L[j] = keep ? st.L[j] : kGuard2;
+    });
+    st.mp = keep ? st.mp : 0u;
+}
+
+__device__ __forceinline__ uint4 add_u16x8(uint4 a, uint4 b)
+{
+    return make_uint4(W2(U2(a.x) + U2(b.x)), W2(U2(a.y) + U2(b.y)), W2(U2(a.z) + U2(b.z)), W2(U2(a.w) + U2(b.w)));
+}
+
+// Family kernel: FAM 0 rows (lanes over rows, steps along x), 1 columns,
+// 2 diagonals (k = x - y), 3 anti-diagonals (s = x + y, last, with the final
+// winner); each chain swept forward (pass 0 direction) then backward.
+template <int CH, int KL, int FAM>
+__global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, int nrig, const int* __restrict__ bs,
+                                            int mode, int keep_sum)
+{
+    extern __shared__ uint32_t lds_s[];   // final sweep: S pairs of the wave's chains
+    constexpr int CPW = 64 / KL;          // chains per wave
+    constexpr int NO = (FAM == 3) ? 3 : 1;
+    const int ri = find_rig(bs, nrig, blockIdx.x);
+    const RigDesc& g = rigs[ri];
+    const int lane = threadIdx.x;
+    const int grp = lane / KL;
+    ChainGeo cg;
+    cg.lid = lane % KL;
+    cg.nq = g.Dp >> 4;
+    cg.half_tail = (g.D & 15) != 0;
+    const int Wp = g.Wp, Hp = g.Hp, Dp = g.Dp, D = g.D;
+    const int c = (blockIdx.x - bs[ri]) * CPW + grp;     // chain index
+    const int nchain = FAM == 0 ? Hp : (FAM == 1 ? Wp : Wp + Hp - 1);
+    const bool valid = c < nchain;
+    const int cw0 = (blockIdx.x - bs[ri]) * CPW;
+    const int cw1 = min(cw0 + CPW - 1, nchain - 1);
+    uint8_t* part = g.part[FAM];
+    uint16_t* edge = g.edge + (size_t)FAM * 2 * (size_t)Wp * Dp;
+    const int boff = cg.lid * CH * 16;     // byte offset of the lane's chunks in a cell
+    const bool final_pass = FAM == 3;
+
+    // ---- chain geometry --------------------------------------------------
+    // steps 0..n-1 of the wave; cell of chain c at step t, per direction
+    int nsteps;
+    int ylo = 0, yhi = 0, ent_dn = 0, ent_up = 0;
+    if (FAM == 0) {
+        nsteps = Wp;
+    } else if (FAM == 1) {
+        ylo = 0; yhi = Hp - 1; ent_dn = 0; ent_up = Hp - 1;
+        nsteps = Hp;
+    } else if (FAM == 2) {
+        const int klo = cw0 - (Hp - 1), khi = cw1 - (Hp - 1);
+        ylo = max(0, -khi); yhi = min(Hp - 1, Wp - 1 - klo);
+        ent_dn = max(0, -klo); ent_up = min(Hp - 1, Wp - 1 - khi);
+        nsteps = yhi - ylo + 1;
+    } else {
+        ylo = max(0, cw0 - (Wp - 1)); yhi = min(Hp - 1, cw1);
+        ent_dn = max(0, cw1 - (Wp - 1)); ent_up = min(Hp - 1, cw0);
+        nsteps = yhi - ylo + 1;
+    }
+    auto cell_of = [&](int dir, int t, int& y, int& x) {
+        if (FAM == 0) { y = c; x = dir == 0 ? t : Wp - 1 - t; return; }
+        y = dir == 0 ? ylo + t : yhi - t;
+        if (FAM == 1) x = c;
+        else if (FAM == 2) x = y + (c - (Hp - 1));
+        else x = c - y;
     };
-    run_chain<J>(g, n, y1 == Hp - 1 ? 0 : 1, [&](int i, int& yy, int& xx) { yy = y1 - i; xx = s - yy; }, final_emit, lane);
+    auto is_active = [&](int y, int x) { return valid && x >= 0 && x < Wp && y >= 0 && y < Hp; };
+
+    const bool top = FAM == 0 && c == 0, bot = FAM == 0 && c == Hp - 1;
+    const bool want_sub = final_pass && mode == 0 && g.subpix != nullptr;
+    uint16_t* outm = g.out[mode];
+
+    LaneState<CH> st;
+    for (int dir = 0; dir < 2; dir++) {
+        lane_guard<CH>(st, false);
+        const bool need_p = dir == 1;
+        const bool need_o = final_pass && dir == 1;
+        auto fetch = [&](int t, RowBuf<CH, NO>& b) {
+            int y, x;
+            cell_of(dir, t, y, x);
+            const bool act = t < nsteps && is_active(y, x);
+            const size_t off = act ? ((size_t)y * Wp + x) * Dp + boff : 0;
+            static_for<0, CH>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const bool ok = act && cg.lid * CH + q < cg.nq;
+                b.m[q] = ok ? *(const uint4*)(g.msum + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                if (need_p) b.p[q] = ok ? *(const uint4*)(part + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                if (need_o) {
+                    b.o[q] = ok ? *(const uint4*)(g.part[0] + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                    if (NO == 3) {
+                        b.o[CH + q] = ok ? *(const uint4*)(g.part[1] + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                        b.o[2 * CH + q] = ok ? *(const uint4*)(g.part[2] + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                    }
+                }
+            });
+        };
+        auto process = [&](int t, RowBuf<CH, NO>& b) {
+            int y, x;
+            cell_of(dir, t, y, x);
+            const bool act = is_active(y, x);
+            const size_t off = act ? ((size_t)y * Wp + x) * Dp + boff : 0;
+            uint32_t m[CH][8], tp[CH][8];
+            static_for<0, CH>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                unpack16(b.m[q], m[q]);
+            });
+            // ---------------- rows family ----------------
+            if (FAM == 0) {
+                const bool first_line = dir == 0 ? top : bot;   // 255 -> 0, 8-bit truncation
+                if (first_line) {
+                    static_for<0, CH>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+#pragma unroll
+                        for (int i = 0; i < 8; i++) m[q][i] = tilde2(m[q][i]);
+                    });
+                }
+                if (t == 0) {
+                    lane_set<CH, KL>(st, cg, m, true);      // L = M (c~ on a first line), contributes
+                    static_for<0, CH>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+#pragma unroll
+                        for (int i = 0; i < 8; i++) tp[q][i] = 0;
+                    });
+                } else {
+                    lane_step<CH, KL>(st, cg, m, tp, first_line);
+                }
+                if (!act) return;
+                const bool edge_row = top || bot;
+                static_for<0, CH>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    if (cg.lid * CH + q >= cg.nq) return;
+                    if (edge_row) {
+                        uint16_t* e = edge + ((size_t)(top ? 0 : 1) * Wp + x) * Dp + (cg.lid * CH + q) * 16;
+                        uint4 a = make_uint4(st.L[8 * q + 0], st.L[8 * q + 1], st.L[8 * q + 2], st.L[8 * q + 3]);
+                        uint4 bb = make_uint4(st.L[8 * q + 4], st.L[8 * q + 5], st.L[8 * q + 6], st.L[8 * q + 7]);
+                        if (cg.half_tail && cg.lid * CH + q == cg.nq - 1) bb = make_uint4(0, 0, 0, 0);
+                        if (dir == 1) {
+                            a = add_u16x8(a, *(const uint4*)e);
+                            bb = add_u16x8(bb, *(const uint4*)(e + 8));
+                        }
+                        *(uint4*)e = a;
+                        *(uint4*)(e + 8) = bb;
+                    } else {
+                        if (dir == 1) {
+                            uint32_t pf[8];
+                            unpack16(b.p[q], pf);
+#pragma unroll
+                            for (int i = 0; i < 8; i++) tp[q][i] = W2(U2(tp[q][i]) + U2(pf[i]));
+                        }
+                        *(uint4*)(part + off + 16 * q) = pack16(tp[q]);
+                    }
+                });
+                return;
+            }
+            // ---------------- vertical families ----------------
+            const int first_row = dir == 0 ? 0 : Hp - 1;
+            if (y == first_row) {
+                // first line of the pass: state c~, no contribution (sgm.cpp:103-207)
+                static_for<0, CH>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+#pragma unroll
+                    for (int i = 0; i < 8; i++) m[q][i] = tilde2(m[q][i]);
+                });
+                lane_set<CH, KL>(st, cg, m, act);
+                if (final_pass && keep_sum && g.sum && act) {
+                    // S of row H'-1 = the four families' exact edge rows
+                    static_for<0, CH>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        if (cg.lid * CH + q >= cg.nq) return;
+                        const size_t eo = (size_t)x * Dp + cg.lid * CH * 16 + 16 * q;
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+                            uint4 acc = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                            for (int fm = 0; fm < 4; fm++)
+                                acc = add_u16x8(acc, *(const uint4*)(g.edge + (size_t)(fm * 2 + 1) * Wp * Dp + eo + 8 * h));
+                            *(uint4*)(g.sum + ((size_t)y * Wp) * Dp + eo + 8 * h) = acc;
+                        }
+                    });
+                }
+                return;
+            }
+            lane_step<CH, KL>(st, cg, m, tp, false);
+            const bool entering = dir == 0 ? (y <= ent_dn) : (y >= ent_up);
+            if (entering) lane_guard<CH>(st, act);
+            const bool edge_row = (y == 0 || y == Hp - 1);
+            const int oy = y - D, ox = x - D;
+            const bool crop = final_pass && act && oy >= 0 && oy < g.H && ox >= 0 && ox < g.W;
+            uint32_t key = 0xFFFFFFFFu;
+            static_for<0, CH>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const int qg = cg.lid * CH + q;
+                if (!act || qg >= cg.nq) return;
+                if (edge_row) {
+                    uint16_t* e = edge + ((size_t)(y == 0 ? 0 : 1) * Wp + x) * Dp + qg * 16;
+                    uint4 a = make_uint4(st.L[8 * q + 0], st.L[8 * q + 1], st.L[8 * q + 2], st.L[8 * q + 3]);
+                    uint4 bb = make_uint4(st.L[8 * q + 4], st.L[8 * q + 5], st.L[8 * q + 6], st.L[8 * q + 7]);
+                    if (cg.half_tail && qg == cg.nq - 1) bb = make_uint4(0, 0, 0, 0);
+                    *(uint4*)e = a;
+                    *(uint4*)(e + 8) = bb;
+                    if (final_pass && keep_sum && g.sum && y == 0) {
+                        // S of row 0: the other families' edge rows + this sweep
+                        const size_t eo = (size_t)x * Dp + qg * 16;
+                        uint4 s0 = a, s1 = bb;
+#pragma unroll
+                        for (int fm = 0; fm < 3; fm++) {
+                            s0 = add_u16x8(s0, *(const uint4*)(g.edge + (size_t)(fm * 2) * Wp * Dp + eo));
+                            s1 = add_u16x8(s1, *(const uint4*)(g.edge + (size_t)(fm * 2) * Wp * Dp + eo + 8));
+                        }
+                        *(uint4*)(g.sum + eo) = s0;
+                        *(uint4*)(g.sum + eo + 8) = s1;
+                    }
+                    return;
+                }
+                if (dir == 0) {
+                    *(uint4*)(part + off + 16 * q) = pack16(tp[q]);
+                    return;
+                }
+                uint32_t pf[8];
+                unpack16(b.p[q], pf);
+#pragma unroll
+                for (int i = 0; i < 8; i++) tp[q][i] = W2(U2(tp[q][i]) + U2(pf[i]));
+                if (!final_pass) {
+                    *(uint4*)(part + off + 16 * q) = pack16(tp[q]);
+                    return;
+                }
+                if (!(crop || keep_sum)) return;
+                // S = 8 M + part[rows] + part[cols] + part[diag] + this family
+                uint32_t pr[8], pc[8], pd[8], S[8];
+                unpack16(b.o[q], pr);
+                unpack16(b.o[NO == 3 ? CH + q : q], pc);
+                unpack16(b.o[NO == 3 ? 2 * CH + q : q], pd);
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const us2 s8 = U2(m[q][i]) << (us2){3, 3};
+                    S[i] = W2(s8 + U2(pr[i]) + U2(pc[i]) + U2(pd[i]) + U2(tp[q][i]));
+                }
+                if (keep_sum && g.sum) {
+                    uint16_t* sc = g.sum + (off - boff) + qg * 16;
+                    *(uint4*)sc = make_uint4(S[0], S[1], S[2], S[3]);
+                    *(uint4*)(sc + 8) = make_uint4(S[4], S[5], S[6], S[7]);
+                }
+                if (!crop) return;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const uint32_t d0 = (uint32_t)(qg * 16 + 2 * i);
+                    const uint32_t k0 = ((S[i] & 0xFFFFu) << 9) | d0;
+                    const uint32_t k1 = ((S[i] >> 16) << 9) | (d0 + 1);
+                    key = min(key, d0 < (uint32_t)D ? k0 : 0xFFFFFFFFu);
+                    key = min(key, d0 + 1 < (uint32_t)D ? k1 : 0xFFFFFFFFu);
+                    if (want_sub) lds_s[(grp * cg.nq * 8) + qg * 8 + i] = S[i];
+                }
+            });
+            if (final_pass && dir == 1) {
+                // the winner of each chain's cell: minimum key over the group
+                key = group_min<KL>(key);
+                if (want_sub) __syncthreads();
+                if (crop && cg.lid == 0) {
+                    const uint32_t best = key & 511u;
+                    const size_t o = (size_t)oy * g.W + ox;
+                    if (outm) outm[o] = (uint16_t)min(best * 255u, 65535u);
+                    if (mode == 0 && g.disp) g.disp[o] = (uint16_t)best;
+                    if (want_sub) {
+                        float delta = 0.0f;
+                        if (best > 0 && (int)best < D - 1) {
+                            const uint16_t* sv = (const uint16_t*)(lds_s + grp * cg.nq * 8);
+                            const int sm = sv[best - 1], s0 = (int)(key >> 9), sp = sv[best + 1];
+                            const int den = sm - 2 * s0 + sp;
+                            if (den > 0) delta = (float)(sm - sp) / (2.0f * (float)den);
+                        }
+                        g.subpix[o] = (float)best + delta;
+                    }
+                }
+                if (want_sub) __syncthreads();
+            }
+        };
+        // software pipeline: the loads of the next cell are in flight while
+        // the current one is computed
+        RowBuf<CH, NO> A, B;
+        fetch(0, A);
+        for (int t = 0; t < nsteps; t += 2) {
+            fetch(t + 1, B);
+            process(t, A);
+            if (t + 1 >= nsteps) break;
+            fetch(t + 2, A);
+            process(t + 1, B);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
-#define SISTER_J_DISPATCH(J, KERNEL, ...)                                                        \
-    switch (J) {                                                                                 \
-    case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                                   \
-    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                                   \
-    case 3: hipLaunchKernelGGL(KERNEL<3>, __VA_ARGS__); break;                                   \
-    case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                                   \
-    case 5: hipLaunchKernelGGL(KERNEL<5>, __VA_ARGS__); break;                                   \
-    case 6: hipLaunchKernelGGL(KERNEL<6>, __VA_ARGS__); break;                                   \
-    case 7: hipLaunchKernelGGL(KERNEL<7>, __VA_ARGS__); break;                                   \
-    default: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                                  \
+// SGM launches: (CH chunks per lane, KL lanes per chain) instantiations
+#define SISTER_FAM_LAUNCH(CHV, KLV, FAM, ...) hipLaunchKernelGGL((k_fam<CHV, KLV, FAM>), __VA_ARGS__)
+#define SISTER_FAM_KL4(CHV, FAM, ...)                                                            \
+    switch (CHV) {                                                                               \
+    case 1: SISTER_FAM_LAUNCH(1, 4, FAM, __VA_ARGS__); break;                                    \
+    case 2: SISTER_FAM_LAUNCH(2, 4, FAM, __VA_ARGS__); break;                                    \
+    case 3: SISTER_FAM_LAUNCH(3, 4, FAM, __VA_ARGS__); break;                                    \
+    case 4: SISTER_FAM_LAUNCH(4, 4, FAM, __VA_ARGS__); break;                                    \
+    case 5: SISTER_FAM_LAUNCH(5, 4, FAM, __VA_ARGS__); break;                                    \
+    case 6: SISTER_FAM_LAUNCH(6, 4, FAM, __VA_ARGS__); break;                                    \
+    default: SISTER_FAM_LAUNCH(8, 4, FAM, __VA_ARGS__); break;                                   \
     }
+#define SISTER_FAM_KL16(CHV, FAM, ...)                                                           \
+    switch (CHV) {                                                                               \
+    case 1: SISTER_FAM_LAUNCH(1, 16, FAM, __VA_ARGS__); break;                                   \
+    default: SISTER_FAM_LAUNCH(2, 16, FAM, __VA_ARGS__); break;                                  \
+    }
+#define SISTER_FAM(G, FAM, ...)                                                                  \
+    do {                                                                                         \
+        if ((G).KL == 16) { SISTER_FAM_KL16((G).CH, FAM, __VA_ARGS__); }                         \
+        else { SISTER_FAM_KL4((G).CH, FAM, __VA_ARGS__); }                                       \
+    } while (0)
+
+int sgm_pick_kl(long cols_chains)
+{
+    // few chains (a single rig): spread each chain over 16 lanes for more
+    // waves in flight; many chains (a batch): 4 lanes, less padding and
+    // cross-lane work per disparity
+    return cols_chains * 4 / 64 >= 1024 ? 4 : 16;
+}
+
+int sgm_pick_ch(int D, int KL)
+{
+    const int k16 = ((D + 15) / 16);
+    int ch = (k16 + KL - 1) / KL;
+    if (KL == 16) return ch <= 1 ? 1 : 2;
+    static const int ladder[] = {1, 2, 3, 4, 5, 6, 8};
+    for (int v : ladder)
+        if (ch <= v) return v;
+    return 8;
+}
+
+void sgm_group_blocks(const RigDesc& g, int KL, int out[3])
+{
+    const int cpw = 64 / KL;
+    out[0] = (g.Hp + cpw - 1) / cpw;
+    out[1] = (g.Wp + cpw - 1) / cpw;
+    out[2] = (g.Wp + g.Hp - 1 + cpw - 1) / cpw;
+}
 
 const char* const kKernelNames[kNumKernels] = {
     "census", "wta", "median_prep", "median_serial", "mask", "msum",
@@ -704,43 +1034,55 @@ void launch_mask(const LaunchSet& ls, int fmask, hipStream_t st, KernelTimer* t)
     TE(kKMask);
 }
 
+size_t msum_lds(int maxD)
+{
+    return (size_t)2 * (kRun + maxD - 1) * 8 + (size_t)2 * kRun * 8 + kRun;
+}
+
 void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t)
 {
+    const size_t lds = msum_lds(ls.maxD);
     TB(kKMsum);
-    hipLaunchKernelGGL(k_msum, dim3(ls.blocks[kStageMsum]), dim3(256), 0, st, ls.rigs, ls.nrig, ls.bs[kStageMsum], fuse);
+    if (fuse & 5)   // frames 0/180 write M (and zero it when they are the only ones)
+        hipLaunchKernelGGL(k_msum_run<0>, dim3(ls.blocks[kStageMsum]), dim3(256), lds, st, ls.rigs, ls.nrig,
+                           ls.bs[kStageMsum], fuse);
+    if (fuse & 10)  // frames 90/270 add (or write when 0/180 are unused)
+        hipLaunchKernelGGL(k_msum_run<1>, dim3(ls.blocks[kStageMsumV]), dim3(256), lds, st, ls.rigs, ls.nrig,
+                           ls.bs[kStageMsumV], fuse);
     TE(kKMsum);
 }
 
 void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, KernelTimer* t)
 {
-    const int J = (ls.maxD + 63) / 64;
-    TB(kKRows);
-    SISTER_J_DISPATCH(J, k_sgm_rows, dim3(ls.blocks[kStageRows]), dim3(256), 0, st, ls.rigs, ls.nrig, ls.bs[kStageRows]);
-    TE(kKRows);
-    TB(kKCols);
-    SISTER_J_DISPATCH(J, k_sgm_cols, dim3(ls.blocks[kStageCols]), dim3(256), 0, st, ls.rigs, ls.nrig, ls.bs[kStageCols]);
-    TE(kKCols);
-    TB(kKDiag);
-    SISTER_J_DISPATCH(J, k_sgm_diag, dim3(ls.blocks[kStageDiag]), dim3(256), 0, st, ls.rigs, ls.nrig, ls.bs[kStageDiag]);
-    TE(kKDiag);
-    TB(kKAnti);
-    SISTER_J_DISPATCH(J, k_sgm_anti, dim3(ls.blocks[kStageDiag]), dim3(256), 0, st, ls.rigs, ls.nrig, ls.bs[kStageDiag],
-                      mode, keep_sum);
-    TE(kKAnti);
+    for (int gi = 0; gi < ls.ngroups; gi++) {
+        const SgmGroup& G = ls.groups[gi];
+        const size_t lds = (size_t)64 * G.CH * 32;
+        TB(kKRows);
+        SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
+        TE(kKRows);
+        TB(kKCols);
+        SISTER_FAM(G, 1, dim3(G.blocks[1]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[1], mode, keep_sum);
+        TE(kKCols);
+        TB(kKDiag);
+        SISTER_FAM(G, 2, dim3(G.blocks[2]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[2], mode, keep_sum);
+        TE(kKDiag);
+        TB(kKAnti);
+        SISTER_FAM(G, 3, dim3(G.blocks[2]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[2], mode, keep_sum);
+        TE(kKAnti);
+    }
 }
 
 // per-rig block counts of each stage (the host builds the block-start tables)
-void stage_blocks(const RigDesc& g, int out[kNumStages])
+void stage_blocks(const RigDesc& g, int maxD, int out[kNumStages])
 {
+    (void)maxD;
     const long npx = (long)g.Wp * g.Hp;
     out[kStageCensus] = 8 * ((g.Wp + 15) / 16) * ((g.Hp + 15) / 16);
     out[kStageWta] = 2 * g.Hp * ((g.Wp + 255) / 256) + 2 * g.Wp * ((g.Hp + 255) / 256);
     out[kStageMedianPrep] = (int)(8 * ((npx + 255) / 256));
     out[kStageMask] = (int)(4 * ((npx + 255) / 256));
-    out[kStageMsum] = (int)((npx + 3) / 4);
-    out[kStageRows] = (g.Hp + 3) / 4;
-    out[kStageCols] = (g.Wp + 3) / 4;
-    out[kStageDiag] = (g.Wp + g.Hp - 1 + 3) / 4;
+    out[kStageMsum] = g.Hp * ((g.Wp + kRun - 1) / kRun);
+    out[kStageMsumV] = g.Wp * ((g.Hp + kRun - 1) / kRun);
 }
 
 }  // namespace sister

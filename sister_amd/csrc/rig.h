@@ -25,14 +25,18 @@ struct RigDesc {
     int W, H, D;        // user rig
     int Wp, Hp;         // padded
     int stride;         // row stride of the input views (bytes)
-    int pad_;
+    int Dp;             // cell stride of the D-deep volumes: D rounded up to 16
     const uint8_t* view[5];   // center, right, top, left, bottom (gray u8, W x H)
     uint64_t* code[8];        // census, frame coords: [2F] center frame, [2F+1] neighbour frame
     uint16_t* wta[8];         // raw WTA: [2F] left map, [2F+1] right map (frame coords)
     uint16_t* med[8];         // in-place-median results of wta[]
     uint8_t* mask[4];         // confidence mask of frame F, padded original coords H' x W'
-    uint8_t* msum;            // mask-weighted cost sum, H' x W' x D (u8: sum <= 4*63)
-    uint16_t* sum;            // SGM aggregate S, H' x W' x D
+    uint8_t* msum;            // mask-weighted cost sum M, H' x W' x Dp (u8: M <= 4*63)
+    uint8_t* part[4];         // per chain family (rows, cols, diag, anti): for rows
+                              // 1..H'-2 the two directions' (L - M) summed, in [0,200]
+    uint16_t* edge;           // exact family contributions on rows 0 and H'-1:
+                              // [family][2][W'][Dp]
+    uint16_t* sum;            // SGM aggregate S, H' x W' x Dp (debug only, may be null)
     uint16_t* out[3];         // API maps W x H, min(255*d, 65535) (hpp:111-118); nullable
     uint16_t* disp;           // integer multiview disparity W x H; nullable
     float* subpix;            // parabola-refined multiview disparity W x H; nullable

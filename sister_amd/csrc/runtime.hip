@@ -37,13 +37,16 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 // workspace layout of one rig (offsets from the workspace base)
 struct WsLayout {
-    size_t code, wta, med, mask, msum, sum, scratch, total;
+    size_t code, wta, med, mask, msum, part, edge, scratch, sum, total;
 };
 
-WsLayout ws_layout(int W, int H, int D)
+inline int padded_depth(int D) { return (D + 15) & ~15; }
+
+// keep_sum adds the debug copy of S
+WsLayout ws_layout(int W, int H, int D, bool keep_sum = false)
 {
     const size_t Wp = (size_t)W + 2 * D, Hp = (size_t)H + 2 * D;
-    const size_t npx = Wp * Hp, ncell = npx * D;
+    const size_t npx = Wp * Hp, ncell = npx * padded_depth(D);
     WsLayout l;
     size_t o = 0;
     l.code = o;    o += align_up(8 * npx * sizeof(uint64_t));
@@ -51,8 +54,10 @@ WsLayout ws_layout(int W, int H, int D)
     l.med = o;     o += align_up(8 * npx * sizeof(uint16_t));
     l.mask = o;    o += align_up(4 * npx);
     l.msum = o;    o += align_up(ncell);
-    l.sum = o;     o += align_up(ncell * sizeof(uint16_t));
+    l.part = o;    o += 4 * align_up(ncell);
+    l.edge = o;    o += align_up(4 * 2 * Wp * padded_depth(D) * sizeof(uint16_t));
     l.scratch = o; o += align_up(8 * npx * sizeof(uint64_t));
+    l.sum = o;     if (keep_sum) o += align_up(ncell * sizeof(uint16_t));
     l.total = o;
     return l;
 }
@@ -172,7 +177,8 @@ struct EventTimer : KernelTimer {
 
 // A prepared launch: rig table and block-start tables in device memory.
 struct Plan {
-    std::vector<RigDesc> host_rigs;
+    std::vector<RigDesc> host_rigs;   // table order (grouped by D)
+    std::vector<int> slot;            // caller's rig index -> table index
     std::vector<Block> workspaces;
     Block table{nullptr, 0};
     LaunchSet ls{};
@@ -237,11 +243,13 @@ static int fail(sister_ctx* c, int code, const std::string& msg)
     } while (0)
 
 // Build the device plan for n rigs whose views/outputs are device pointers.
+// The rig table is ordered by D so that the SGM launches can take each group
+// of equal depth as one contiguous slice.
 static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
 {
     p.host_rigs.clear();
     p.workspaces.clear();
-    int maxD = 8, maxCols = 8;
+    p.slot.assign(n, 0);
     for (int i = 0; i < n; i++) {
         const sister_rig& r = rigs[i];
         if (!valid_dims(r.width, r.height, r.disp_count))
@@ -249,12 +257,22 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         for (int v = 0; v < 5; v++)
             if (!r.view[v]) return fail(c, SISTER_EINVAL, "rig " + std::to_string(i) + ": null view");
         if (r.stride < r.width) return fail(c, SISTER_EINVAL, "rig " + std::to_string(i) + ": stride < width");
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rigs[a].disp_count < rigs[b].disp_count; });
+    int maxD = 8, maxCols = 8;
+    for (int k = 0; k < n; k++) {
+        const int i = order[k];
+        p.slot[i] = k;
+        const sister_rig& r = rigs[i];
         RigDesc g{};
         g.W = r.width; g.H = r.height; g.D = r.disp_count;
         g.Wp = g.W + 2 * g.D; g.Hp = g.H + 2 * g.D;
+        g.Dp = padded_depth(g.D);
         g.stride = r.stride;
         for (int v = 0; v < 5; v++) g.view[v] = r.view[v];
-        WsLayout l = ws_layout(g.W, g.H, g.D);
+        WsLayout l = ws_layout(g.W, g.H, g.D, c->keep_sum);
         Block b;
         if (c->pool.get(l.total, b) != hipSuccess) {
             for (auto& w : p.workspaces) c->pool.put(w);
@@ -263,14 +281,17 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         }
         p.workspaces.push_back(b);
         const size_t npx = (size_t)g.Wp * g.Hp;
-        for (int k = 0; k < 8; k++) {
-            g.code[k] = (uint64_t*)(b.ptr + l.code) + k * npx;
-            g.wta[k] = (uint16_t*)(b.ptr + l.wta) + k * npx;
-            g.med[k] = (uint16_t*)(b.ptr + l.med) + k * npx;
+        for (int q = 0; q < 8; q++) {
+            g.code[q] = (uint64_t*)(b.ptr + l.code) + q * npx;
+            g.wta[q] = (uint16_t*)(b.ptr + l.wta) + q * npx;
+            g.med[q] = (uint16_t*)(b.ptr + l.med) + q * npx;
         }
-        for (int k = 0; k < 4; k++) g.mask[k] = (uint8_t*)(b.ptr + l.mask) + k * npx;
+        for (int q = 0; q < 4; q++) g.mask[q] = (uint8_t*)(b.ptr + l.mask) + q * npx;
+        const size_t ncell = npx * g.Dp;
         g.msum = (uint8_t*)(b.ptr + l.msum);
-        g.sum = (uint16_t*)(b.ptr + l.sum);
+        for (int q = 0; q < 4; q++) g.part[q] = (uint8_t*)(b.ptr + l.part) + q * align_up(ncell);
+        g.edge = (uint16_t*)(b.ptr + l.edge);
+        g.sum = c->keep_sum ? (uint16_t*)(b.ptr + l.sum) : nullptr;
         g.out[0] = r.out_multiview;
         g.out[1] = r.out_horizontal;
         g.out[2] = r.out_vertical;
@@ -280,28 +301,65 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         maxCols = std::max(maxCols, std::max(g.Wp, g.Hp));
         p.host_rigs.push_back(g);
     }
-    // table: rigs | scratch ptrs | bs[kNumStages][n+1]
+    // SGM groups: runs of equal D
+    struct GroupHost { int i0, i1, KL, CH; };
+    std::vector<GroupHost> groups;
+    for (int k = 0; k < n;) {
+        int e = k;
+        long chains = 0;
+        while (e < n && p.host_rigs[e].D == p.host_rigs[k].D) chains += p.host_rigs[e++].Wp;
+        const int KL = sgm_pick_kl(chains);
+        groups.push_back({k, e, KL, sgm_pick_ch(p.host_rigs[k].D, KL)});
+        k = e;
+    }
+    if ((int)groups.size() > kMaxSgmGroups) {
+        for (auto& w : p.workspaces) c->pool.put(w);
+        p.workspaces.clear();
+        return fail(c, SISTER_EINVAL, "too many distinct disparity counts in one batch");
+    }
+    // table: rigs | scratch ptrs | bs[kNumStages][n+1] | per group 3 x (ng+1)
     const size_t rig_bytes = align_up(sizeof(RigDesc) * n);
     const size_t scr_bytes = align_up(sizeof(uint64_t*) * n);
     const size_t bs_bytes = align_up(sizeof(int) * (n + 1) * kNumStages);
-    std::vector<char> host(rig_bytes + scr_bytes + bs_bytes, 0);
+    size_t gbs_ints = 0;
+    for (auto& G : groups) gbs_ints += 3 * (size_t)(G.i1 - G.i0 + 1);
+    const size_t gbs_bytes = align_up(sizeof(int) * gbs_ints);
+    std::vector<char> host(rig_bytes + scr_bytes + bs_bytes + gbs_bytes, 0);
     std::memcpy(host.data(), p.host_rigs.data(), sizeof(RigDesc) * n);
     uint64_t** scr = (uint64_t**)(host.data() + rig_bytes);
     int* bs = (int*)(host.data() + rig_bytes + scr_bytes);
-    for (int i = 0; i < n; i++) {
-        WsLayout l = ws_layout(p.host_rigs[i].W, p.host_rigs[i].H, p.host_rigs[i].D);
-        scr[i] = (uint64_t*)(p.workspaces[i].ptr + l.scratch);
+    int* gbs = (int*)(host.data() + rig_bytes + scr_bytes + bs_bytes);
+    for (int k = 0; k < n; k++) {
+        WsLayout l = ws_layout(p.host_rigs[k].W, p.host_rigs[k].H, p.host_rigs[k].D, c->keep_sum);
+        scr[k] = (uint64_t*)(p.workspaces[k].ptr + l.scratch);
     }
-    for (int s = 0; s < kNumStages; s++) p.ls.blocks[s] = 0;
-    for (int i = 0; i < n; i++) {
+    for (int st = 0; st < kNumStages; st++) p.ls.blocks[st] = 0;
+    for (int k = 0; k < n; k++) {
         int nb[kNumStages];
-        stage_blocks(p.host_rigs[i], nb);
-        for (int s = 0; s < kNumStages; s++) {
-            bs[s * (n + 1) + i] = p.ls.blocks[s];
-            p.ls.blocks[s] += nb[s];
+        stage_blocks(p.host_rigs[k], maxD, nb);
+        for (int st = 0; st < kNumStages; st++) {
+            bs[st * (n + 1) + k] = p.ls.blocks[st];
+            p.ls.blocks[st] += nb[st];
         }
     }
-    for (int s = 0; s < kNumStages; s++) bs[s * (n + 1) + n] = p.ls.blocks[s];
+    for (int st = 0; st < kNumStages; st++) bs[st * (n + 1) + n] = p.ls.blocks[st];
+    size_t gpos = 0;
+    std::vector<size_t> gofs;
+    for (auto& G : groups) {
+        const int ng = G.i1 - G.i0;
+        gofs.push_back(gpos);
+        int tot[3] = {0, 0, 0};
+        for (int k = 0; k < ng; k++) {
+            int nb[3];
+            sgm_group_blocks(p.host_rigs[G.i0 + k], G.KL, nb);
+            for (int f = 0; f < 3; f++) {
+                gbs[gpos + f * (ng + 1) + k] = tot[f];
+                tot[f] += nb[f];
+            }
+        }
+        for (int f = 0; f < 3; f++) gbs[gpos + f * (ng + 1) + ng] = tot[f];
+        gpos += 3 * (size_t)(ng + 1);
+    }
     if (c->pool.get(host.size(), p.table) != hipSuccess) {
         for (auto& w : p.workspaces) c->pool.put(w);
         p.workspaces.clear();
@@ -311,10 +369,25 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
     p.ls.rigs = (const RigDesc*)p.table.ptr;
     p.ls.nrig = n;
     p.ls.scratch = (uint64_t* const*)(p.table.ptr + rig_bytes);
-    for (int s = 0; s < kNumStages; s++)
-        p.ls.bs[s] = (const int*)(p.table.ptr + rig_bytes + scr_bytes) + s * (n + 1);
+    for (int st = 0; st < kNumStages; st++)
+        p.ls.bs[st] = (const int*)(p.table.ptr + rig_bytes + scr_bytes) + st * (n + 1);
     p.ls.maxD = maxD;
     p.ls.maxCols = maxCols;
+    p.ls.ngroups = (int)groups.size();
+    const int* dg = (const int*)(p.table.ptr + rig_bytes + scr_bytes + bs_bytes);
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const GroupHost& G = groups[gi];
+        const int ng = G.i1 - G.i0;
+        SgmGroup& sg = p.ls.groups[gi];
+        sg.rigs = p.ls.rigs + G.i0;
+        sg.nrig = ng;
+        for (int f = 0; f < 3; f++) {
+            sg.bs[f] = dg + gofs[gi] + f * (ng + 1);
+            sg.blocks[f] = gbs[gofs[gi] + f * (ng + 1) + ng];
+        }
+        sg.KL = G.KL;
+        sg.CH = G.CH;
+    }
     return SISTER_OK;
 }
 
@@ -376,6 +449,12 @@ int sister_create(const sister_config* cfg, sister_ctx** out)
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return SISTER_EDEVICE;
+    }
+    if (!(cfg && cfg->hbm_budget)) {
+        // default: 90% of the HBM free at creation
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+            c->pool.budget = free_b / 10 * 9;
     }
     *out = c;
     return SISTER_OK;
@@ -546,7 +625,7 @@ int sister_debug_fetch(sister_ctx* c, int index, const char* what, int which, vo
 {
     if (!c || !what || !dst) return fail(c, SISTER_EINVAL, "bad arguments");
     if (index < 0 || index >= (int)c->last.host_rigs.size()) return fail(c, SISTER_EINVAL, "no such rig in the last run");
-    const RigDesc& g = c->last.host_rigs[index];
+    const RigDesc& g = c->last.host_rigs[c->last.slot[index]];
     const size_t npx = (size_t)g.Wp * g.Hp;
     const void* src = nullptr;
     size_t n = 0;
@@ -555,8 +634,11 @@ int sister_debug_fetch(sister_ctx* c, int index, const char* what, int which, vo
     else if (w == "wta" && which >= 0 && which < 8) { src = g.wta[which]; n = npx * 2; }
     else if (w == "med" && which >= 0 && which < 8) { src = g.med[which]; n = npx * 2; }
     else if (w == "mask" && which >= 0 && which < 4) { src = g.mask[which]; n = npx; }
-    else if (w == "msum") { src = g.msum; n = npx * g.D; }
-    else if (w == "sum") { src = g.sum; n = npx * g.D * 2; }
+    else if (w == "msum") { src = g.msum; n = npx * g.Dp; }
+    else if (w == "sum") {
+        if (!g.sum) return fail(c, SISTER_EINVAL, "S is only kept after sister_debug_keep_sum(ctx, 1)");
+        src = g.sum; n = npx * g.Dp * 2;
+    }
     else return fail(c, SISTER_EINVAL, "unknown intermediate " + w);
     if (bytes < n) return fail(c, SISTER_EINVAL, "destination too small");
     HIPCHK(c, hipStreamSynchronize(c->stream));

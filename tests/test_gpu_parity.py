@@ -51,10 +51,11 @@ def _compare_stages(ctx, O, W, H, D, seed, modes=7):
             bad.append(("mask", F, int((g != ref["mask"][F]).sum())))
     # the workspace holds the last mode's cost sum and aggregate
     last = max(m for m in range(3) if modes & (1 << m))
-    g = ctx.fetch(0, "msum", 0, (Hp, Wp, D), np.uint8)
+    Dp = (D + 15) // 16 * 16
+    g = ctx.fetch(0, "msum", 0, (Hp, Wp, Dp), np.uint8)[:, :, :D]
     if not np.array_equal(g.astype(np.uint16), ref["msum"][last]):
         bad.append(("msum", int((g != ref["msum"][last]).sum())))
-    g = ctx.fetch(0, "sum", 0, (Hp, Wp, D), np.uint16)
+    g = ctx.fetch(0, "sum", 0, (Hp, Wp, Dp), np.uint16)[:, :, :D]
     if not np.array_equal(g, ref["sum"][last]):
         diff = np.argwhere(g != ref["sum"][last])
         bad.append(("sum", len(diff), diff[:3].tolist()))
