@@ -86,6 +86,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run `python -m sister_amd.build`")
+        # PyTorch-ROCm ships its own libamdhip64 with the same soname as the
+        # system one.  Loading torch first makes this library bind to the
+        # runtime torch uses, so device tensors and this library share one
+        # HIP runtime in the process (two copies cannot both own the GPU).
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         l = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(l, name)

@@ -47,7 +47,22 @@ def build(verbose=False):
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+CLI = os.path.join(HERE, "compute_disp")
+
+
+def build_cli(verbose=False):
+    """compute_disp: host C++ over the C ABI (g++, zlib)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "cli", "compute_disp.cpp"),
+           "-I", os.path.join(ROOT, "include"), "-L", HERE, "-lsister_hip", "-lz",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", CLI]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return CLI
 
 
 if __name__ == "__main__":

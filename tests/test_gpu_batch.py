@@ -1,0 +1,183 @@
+"""GPU: batched plans, graph replay, serving, error paths, full-size parity.
+
+Every result is compared bit-exactly with the oracle (small rigs) or with the
+reference's own kernels (oracle/_ref, the 1920x1080 D=192 rig), or with the
+same rig computed alone (size-independent properties).
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import sister_amd
+from sister_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+MIX = [(64, 48, 16, 51), (96, 64, 24, 52), (80, 60, 72, 53), (128, 96, 32, 54), (64, 48, 16, 55), (72, 40, 64, 56)]
+
+
+def _device_rig(ctx, rig, torch):
+    H, W = rig["views"][0].shape
+    views = [torch.from_numpy(v).cuda() for v in rig["views"]]
+    outs = {"multiview": torch.zeros((H, W), dtype=torch.uint16, device="cuda"),
+            "disp": torch.zeros((H, W), dtype=torch.uint16, device="cuda"),
+            "subpix": torch.zeros((H, W), dtype=torch.float32, device="cuda")}
+    return dict(views=views, outs=outs, struct=ctx.rig_struct(views, rig["D"], outs))
+
+
+def test_batched_mixed_sizes_match_oracle(gpu_ctx, oracle_mod):
+    import torch
+    rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in MIX]
+    dev = [_device_rig(gpu_ctx, r, torch) for r in rigs]
+    print("[batch] 6 mixed rigs in one plan", flush=True)
+    gpu_ctx.run_device([d["struct"] for d in dev], sister_amd.MODE_MULTIVIEW)
+    gpu_ctx.sync()
+    for r, d in zip(rigs, dev):
+        ref = oracle_mod.compute(r["views"], r["D"], modes=1)
+        assert np.array_equal(d["outs"]["multiview"].cpu().numpy(), ref["multiview"])
+        assert np.array_equal(d["outs"]["disp"].cpu().numpy(), ref["disp"])
+        assert float(np.abs(d["outs"]["subpix"].cpu().numpy() - ref["subpix"]).max()) <= 1e-3
+
+
+def test_graph_replays_match_eager(gpu_ctx):
+    import torch
+    rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in MIX[:4]]
+    dev = [_device_rig(gpu_ctx, r, torch) for r in rigs]
+    gpu_ctx.run_device([d["struct"] for d in dev], 1)
+    gpu_ctx.sync()
+    eager = [d["outs"]["disp"].cpu().numpy().copy() for d in dev]
+    other = _device_rig(gpu_ctx, synth.rig_small(64, 48, 16, 99), torch)
+    print("[graph] capture 4 rigs, replay 3x with eager work between", flush=True)
+    gpu_ctx.graph_capture([d["struct"] for d in dev], 1)
+    for rep in range(3):
+        for d in dev:
+            d["outs"]["disp"].zero_()
+        torch.cuda.synchronize()
+        gpu_ctx.run_device([other["struct"]], 1)     # unrelated eager work in between
+        gpu_ctx.sync()
+        gpu_ctx.graph_launch()
+        gpu_ctx.sync()
+        for e, d in zip(eager, dev):
+            assert np.array_equal(d["outs"]["disp"].cpu().numpy(), e), rep
+
+
+def test_deterministic_and_plain_equals_batched(gpu_ctx):
+    import torch
+    rig = synth.rig_small(160, 120, 40, 61)
+    a = _device_rig(gpu_ctx, rig, torch)
+    gpu_ctx.run_device([a["struct"]], 1)
+    gpu_ctx.sync()
+    first = a["outs"]["disp"].cpu().numpy().copy()
+    gpu_ctx.run_device([a["struct"]], 1)
+    gpu_ctx.sync()
+    assert np.array_equal(a["outs"]["disp"].cpu().numpy(), first)
+    # the same rig inside a larger batch (different launch geometry)
+    others = [_device_rig(gpu_ctx, synth.rig_small(W, H, D, s), torch) for W, H, D, s in MIX]
+    b = _device_rig(gpu_ctx, rig, torch)
+    gpu_ctx.run_device([o["struct"] for o in others] + [b["struct"]], 1)
+    gpu_ctx.sync()
+    assert np.array_equal(b["outs"]["disp"].cpu().numpy(), first)
+
+
+def _host_rig_struct(r, outs):
+    s = sister_amd.RigStruct()
+    s.width, s.height, s.stride, s.disp_count = r["W"], r["H"], r["W"], r["D"]
+    for k in range(5):
+        s.view[k] = r["views"][k].ctypes.data
+    s.out_multiview = outs["multiview"].ctypes.data
+    s.out_disp = outs["disp"].ctypes.data
+    return s
+
+
+def test_serving_alone_equals_mix():
+    ctx = sister_amd.Context()
+    try:
+        ctx.serve_start(3)
+        rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in MIX]
+        keep = []
+
+        def submit(r):
+            o = {"multiview": np.zeros((r["H"], r["W"]), np.uint16), "disp": np.zeros((r["H"], r["W"]), np.uint16)}
+            s = _host_rig_struct(r, o)
+            keep.append((s, o, r))
+            return ctx.submit(s, 1), o
+
+        print("[serve] rig alone", flush=True)
+        t, alone = submit(rigs[0])
+        ctx.wait(t)
+        print("[serve] same rig in a mix of 12 arrivals", flush=True)
+        pending = []
+        for i in range(12):
+            pending.append(submit(rigs[i % len(rigs)]))
+            time.sleep(0.001)
+        for t, o in pending:
+            ctx.wait(t)
+        for i, (t, o) in enumerate(pending):
+            if i % len(rigs) == 0:
+                assert np.array_equal(o["disp"], alone["disp"])
+                assert np.array_equal(o["multiview"], alone["multiview"])
+        # every rig of the mix equals its own single run
+        for i, r in enumerate(rigs):
+            single = ctx.compute_rig(r["views"], r["D"], modes=1, subpix=False)
+            assert np.array_equal(pending[i][1]["disp"], single["disp"]), i
+        st = ctx.stats()
+        assert st["completed"] == 13 and st["rejected"] == 0
+        assert st["p99_ms"] >= st["p50_ms"] > 0
+        ctx.serve_stop()
+    finally:
+        ctx.close()
+
+
+def test_serving_admission_by_budget():
+    small = synth.rig_small(64, 48, 16, 71)
+    big = synth.rig_small(320, 240, 64, 72)
+    fp_small = sister_amd.workspace_bytes(64, 48, 16)
+    ctx = sister_amd.Context(hbm_budget=int(fp_small * 3.5))
+    try:
+        ctx.serve_start(2)
+        o = {"multiview": np.zeros((240, 320), np.uint16), "disp": np.zeros((240, 320), np.uint16)}
+        with pytest.raises(sister_amd.SisterError) as e:
+            ctx.submit(_host_rig_struct(big, o), 1)
+        assert e.value.code == sister_amd.SISTER_EBUSY
+        outs = []
+        tickets = []
+        for i in range(8):   # more than fit at once: admitted as memory frees
+            oo = {"multiview": np.zeros((48, 64), np.uint16), "disp": np.zeros((48, 64), np.uint16)}
+            s = _host_rig_struct(small, oo)
+            outs.append((s, oo))
+            tickets.append(ctx.submit(s, 1))
+        for t in tickets:
+            ctx.wait(t)
+        first = outs[0][1]["disp"]
+        assert all(np.array_equal(oo["disp"], first) for _, oo in outs)
+        st = ctx.stats()
+        assert st["completed"] == 8 and st["rejected"] == 1
+        assert st["hbm_peak"] <= int(fp_small * 3.5)
+        ctx.serve_stop()
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("W,H,D", [(64, 48, 60), (66, 48, 16), (64, 46, 16), (64, 48, 0), (64, 48, 520)])
+def test_invalid_rigs_rejected(gpu_ctx, W, H, D):
+    views = [np.zeros((H, W), np.uint8)] * 5
+    with pytest.raises(sister_amd.SisterError) as e:
+        gpu_ctx.compute_rig(views, D, modes=1)
+    assert e.value.code == sister_amd.SISTER_EINVAL
+
+
+def test_full_1080_d192_bit_exact_vs_reference_kernels(gpu_ctx, oracle_mod):
+    if not oracle_mod.ref_available():
+        pytest.skip("oracle/_ref not built")
+    rig = synth.rig_plain_1080()
+    print("[full] 1920x1080 D=192 seed=2 vs reference kernels", flush=True)
+    got = gpu_ctx.compute_rig(rig["views"], rig["D"], modes=1)
+    ref = oracle_mod.compute(rig["views"], rig["D"], modes=1, use_ref=True, subpix=False)
+    assert np.array_equal(got["disp"], ref["disp"])
+    assert np.array_equal(got["multiview"], ref["multiview"])
+    # size-independent properties of the full-size output
+    assert got["disp"].max() <= rig["D"] - 1
+    assert np.array_equal(got["multiview"].astype(np.uint32), np.minimum(255 * got["disp"].astype(np.uint32), 65535))
+    assert np.all(np.abs(got["subpix"] - got["disp"]) <= 0.5 + 1e-6)
