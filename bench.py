@@ -42,6 +42,16 @@ HBM_PEAK = 8.0e12
 # fused into sgm_anti.  Total 28 + 4 + 3*16 + 17 = 97.
 OPS_PER_CELL = {"wta": 28, "msum": 4, "sgm_rows": 16, "sgm_cols": 16, "sgm_diag": 16, "sgm_anti": 17}
 
+# Algorithmic HBM bytes per padded cell of the HBM-bound kernels.  Each SGM
+# chain family reads the u8 cost M in both sweeps (2), writes its forward
+# (L - M) byte (1), reads it back and writes the two-direction sum (2): 5.
+# The anti-diagonal family, which forms S and the winner, instead reads the
+# three other families' bytes (3) and writes nothing in its backward sweep: 7.
+# The cost sum writes M once from the frame-0/180 pass (1) and reads and
+# rewrites it in the frame-90/270 pass (2): 3.
+BYTES_PER_CELL = {"sgm_rows": 5, "sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "msum": 3}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch16_rig1080_d192.json")
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -97,6 +107,23 @@ def cpu_baseline(rig):
     }
 
 
+def run_single(args):
+    """One eager run of the batch: exactly one dispatch per kernel (for
+    hardware-counter runs under rocprofv3 --pmc)."""
+    import torch
+    import sister_amd
+
+    ctx = sister_amd.Context(device=0)
+    rigs = make_rigs(args.workload, args.batch)
+    dev = to_device(ctx, rigs, torch)
+    torch.cuda.synchronize()
+    ctx.run_device([d["struct"] for d in dev], 1)
+    ctx.sync()
+    cells = sum(padded_cells(r["W"], r["H"], r["D"]) for r in rigs)
+    print(json.dumps({"single_run_rigs": len(rigs), "padded_cells": cells}))
+    ctx.close()
+
+
 def run(args):
     import numpy as np
     import torch
@@ -150,8 +177,24 @@ def run(args):
     prof = ctx.profile_read()
     ctx.profile(False)
     kern = {k: ms / n for k, (ms, n) in prof.items() if n}
-    dom = max((k for k in kern if k in OPS_PER_CELL), key=lambda k: kern[k])
-    achieved = OPS_PER_CELL[dom] * cells / (kern[dom] * 1e-3)
+    dom = max(kern, key=lambda k: kern[k])
+    if dom in BYTES_PER_CELL:
+        achieved = BYTES_PER_CELL[dom] * cells / (kern[dom] * 1e-3)
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            t = json.load(open(TRAFFIC_FILE))
+            matches = (args.workload == "rig1080" and B == t.get("rigs"))
+            k = t["kernels"].get(dom)
+            if matches and k:
+                traffic = k["fetch_bytes"] + k["write_bytes"]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                    "bytes_per_cell": BYTES_PER_CELL[dom]}
+    else:
+        achieved = OPS_PER_CELL.get(dom, 0) * cells / (kern[dom] * 1e-3)
+        roofline = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
+                    "unit": "T int-lane-ops/s", "frac": achieved / VALU_PEAK, "traffic": None,
+                    "ops_per_cell": OPS_PER_CELL.get(dom, 0)}
 
     out = {
         "metric": METRIC,
@@ -181,16 +224,7 @@ def run(args):
         "batched_equals_plain": bool(same),
         "padded_cells_per_step": cells,
         "pipeline_valu_frac": 97 * cells / (step_ms * 1e-3) / VALU_PEAK,
-        "roofline": {
-            "bound": "valu",
-            "kernel": dom,
-            "achieved": achieved / 1e12,
-            "peak": VALU_PEAK / 1e12,
-            "unit": "T int-lane-ops/s",
-            "frac": achieved / VALU_PEAK,
-            "ops_per_cell": OPS_PER_CELL[dom],
-            "traffic": None,
-        },
+        "roofline": roofline,
         "kernel_ms": kern,
     }
     if not args.no_cpu_baseline:
@@ -244,6 +278,20 @@ def run_serve(args):
     }
 
 
+def calibrate_copy(gib=1):
+    """A plain streaming kernel of known traffic (reads and writes `gib` GiB)
+    for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = (gib << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.add(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_read": n * 4, "calibrate_copy_bytes_written": n * 4, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,7 +303,15 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--hbm-gb", type=int, default=64)
+    ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--single", action="store_true", help="one eager run of the batch (counter runs)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.single:
+        run_single(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (rigs are independent, "
                                    "a multi-GPU run would shard rigs across processes)"}))
