@@ -20,9 +20,9 @@ enum Stage {
 struct SgmGroup {
     const RigDesc* rigs;     // device pointer to the group's first rig
     int nrig;
-    const int* bs[3];        // block starts: rows, columns, diagonals (+anti)
-    int blocks[3];
-    int CH, KL;
+    const int* bs[4];        // block starts per family: rows, columns, diagonals, anti-diagonals
+    int blocks[4];
+    int CH[4], KL[4];        // chunks per lane, lanes per chain, per family
 };
 constexpr int kMaxSgmGroups = 16;
 
@@ -54,9 +54,9 @@ struct KernelTimer {
 };
 
 void stage_blocks(const RigDesc& g, int maxD, int out[kNumStages]);
-int sgm_pick_kl(long cols_chains);
+int sgm_pick_kl(long cols_chains, int fam, int D);
 int sgm_pick_ch(int D, int KL);
-void sgm_group_blocks(const RigDesc& g, int KL, int out[3]);
+int sgm_family_blocks(const RigDesc& g, int KL, int fam);
 void launch_census(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t);

@@ -16,6 +16,7 @@
 // cover many rigs of different sizes: block b belongs to rig i when
 // bs[i] <= b < bs[i+1].
 #include <hip/hip_runtime.h>
+#include <cstdlib>
 #include "rig.h"
 #include "kernels.h"
 
@@ -926,7 +927,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
             }
         };
         // software pipeline: the loads of the next cell are in flight while
-        // the current one is computed
+        // the current one is computed (deeper prefetch measured no faster)
         RowBuf<CH, NO> A, B;
         fetch(0, A);
         for (int t = 0; t < nsteps; t += 2) {
@@ -949,10 +950,14 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
     case 1: SISTER_FAM_LAUNCH(1, 4, FAM, __VA_ARGS__); break;                                    \
     case 2: SISTER_FAM_LAUNCH(2, 4, FAM, __VA_ARGS__); break;                                    \
     case 3: SISTER_FAM_LAUNCH(3, 4, FAM, __VA_ARGS__); break;                                    \
-    case 4: SISTER_FAM_LAUNCH(4, 4, FAM, __VA_ARGS__); break;                                    \
-    case 5: SISTER_FAM_LAUNCH(5, 4, FAM, __VA_ARGS__); break;                                    \
-    case 6: SISTER_FAM_LAUNCH(6, 4, FAM, __VA_ARGS__); break;                                    \
-    default: SISTER_FAM_LAUNCH(8, 4, FAM, __VA_ARGS__); break;                                   \
+    default: SISTER_FAM_LAUNCH(4, 4, FAM, __VA_ARGS__); break;                                   \
+    }
+#define SISTER_FAM_KL8(CHV, FAM, ...)                                                            \
+    switch (CHV) {                                                                               \
+    case 1: SISTER_FAM_LAUNCH(1, 8, FAM, __VA_ARGS__); break;                                    \
+    case 2: SISTER_FAM_LAUNCH(2, 8, FAM, __VA_ARGS__); break;                                    \
+    case 3: SISTER_FAM_LAUNCH(3, 8, FAM, __VA_ARGS__); break;                                    \
+    default: SISTER_FAM_LAUNCH(4, 8, FAM, __VA_ARGS__); break;                                   \
     }
 #define SISTER_FAM_KL16(CHV, FAM, ...)                                                           \
     switch (CHV) {                                                                               \
@@ -961,16 +966,27 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
     }
 #define SISTER_FAM(G, FAM, ...)                                                                  \
     do {                                                                                         \
-        if ((G).KL == 16) { SISTER_FAM_KL16((G).CH, FAM, __VA_ARGS__); }                         \
-        else { SISTER_FAM_KL4((G).CH, FAM, __VA_ARGS__); }                                       \
+        if ((G).KL[FAM] == 16) { SISTER_FAM_KL16((G).CH[FAM], FAM, __VA_ARGS__); }               \
+        else if ((G).KL[FAM] == 8) { SISTER_FAM_KL8((G).CH[FAM], FAM, __VA_ARGS__); }            \
+        else { SISTER_FAM_KL4((G).CH[FAM], FAM, __VA_ARGS__); }                                  \
     } while (0)
 
-int sgm_pick_kl(long cols_chains)
+int sgm_pick_kl(long cols_chains, int fam, int D)
 {
-    // few chains (a single rig): spread each chain over 16 lanes for more
-    // waves in flight; many chains (a batch): 4 lanes, less padding and
-    // cross-lane work per disparity
-    return cols_chains * 4 / 64 >= 1024 ? 4 : 16;
+    // SISTER_SGM_LANES (4, 8 or 16) overrides the choice, for tuning; 4 lanes
+    // hold at most 256 disparities (beyond that the state would spill)
+    static const int forced = [] {
+        const char* e = getenv("SISTER_SGM_LANES");
+        const int v = e ? atoi(e) : 0;
+        return (v == 4 || v == 8 || v == 16) ? v : 0;
+    }();
+    if (forced) return (forced == 4 && D > 256) ? 8 : forced;
+    // A single rig has few chains: 16 lanes per chain keep the most waves in
+    // flight.  In a batch (measured on 16 rigs of 2304x1464x192 cells) rows
+    // and columns are still fastest with 16 lanes per chain, the diagonal
+    // families, whose final sweep carries the most registers, with 8.
+    if (cols_chains * 4 / 64 < 1024) return 16;
+    return fam >= 2 ? 8 : 16;
 }
 
 int sgm_pick_ch(int D, int KL)
@@ -978,18 +994,14 @@ int sgm_pick_ch(int D, int KL)
     const int k16 = ((D + 15) / 16);
     int ch = (k16 + KL - 1) / KL;
     if (KL == 16) return ch <= 1 ? 1 : 2;
-    static const int ladder[] = {1, 2, 3, 4, 5, 6, 8};
-    for (int v : ladder)
-        if (ch <= v) return v;
-    return 8;
+    return ch <= 3 ? ch : 4;   // KL 8 (D <= 512) and KL 4 (D <= 256)
 }
 
-void sgm_group_blocks(const RigDesc& g, int KL, int out[3])
+int sgm_family_blocks(const RigDesc& g, int KL, int fam)
 {
     const int cpw = 64 / KL;
-    out[0] = (g.Hp + cpw - 1) / cpw;
-    out[1] = (g.Wp + cpw - 1) / cpw;
-    out[2] = (g.Wp + g.Hp - 1 + cpw - 1) / cpw;
+    const int chains = fam == 0 ? g.Hp : (fam == 1 ? g.Wp : g.Wp + g.Hp - 1);
+    return (chains + cpw - 1) / cpw;
 }
 
 const char* const kKernelNames[kNumKernels] = {
@@ -1056,7 +1068,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, Ker
 {
     for (int gi = 0; gi < ls.ngroups; gi++) {
         const SgmGroup& G = ls.groups[gi];
-        const size_t lds = (size_t)64 * G.CH * 32;
+        const size_t lds = (size_t)64 * G.CH[3] * 32;
         TB(kKRows);
         SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
         TE(kKRows);
@@ -1067,7 +1079,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, Ker
         SISTER_FAM(G, 2, dim3(G.blocks[2]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[2], mode, keep_sum);
         TE(kKDiag);
         TB(kKAnti);
-        SISTER_FAM(G, 3, dim3(G.blocks[2]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[2], mode, keep_sum);
+        SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[3], mode, keep_sum);
         TE(kKAnti);
     }
 }

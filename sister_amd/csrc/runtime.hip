@@ -302,14 +302,18 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         p.host_rigs.push_back(g);
     }
     // SGM groups: runs of equal D
-    struct GroupHost { int i0, i1, KL, CH; };
+    struct GroupHost { int i0, i1, KL[4], CH[4]; };
     std::vector<GroupHost> groups;
     for (int k = 0; k < n;) {
         int e = k;
         long chains = 0;
         while (e < n && p.host_rigs[e].D == p.host_rigs[k].D) chains += p.host_rigs[e++].Wp;
-        const int KL = sgm_pick_kl(chains);
-        groups.push_back({k, e, KL, sgm_pick_ch(p.host_rigs[k].D, KL)});
+        GroupHost G{k, e, {}, {}};
+        for (int f = 0; f < 4; f++) {
+            G.KL[f] = sgm_pick_kl(chains, f, p.host_rigs[k].D);
+            G.CH[f] = sgm_pick_ch(p.host_rigs[k].D, G.KL[f]);
+        }
+        groups.push_back(G);
         k = e;
     }
     if ((int)groups.size() > kMaxSgmGroups) {
@@ -317,12 +321,12 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         p.workspaces.clear();
         return fail(c, SISTER_EINVAL, "too many distinct disparity counts in one batch");
     }
-    // table: rigs | scratch ptrs | bs[kNumStages][n+1] | per group 3 x (ng+1)
+    // table: rigs | scratch ptrs | bs[kNumStages][n+1] | per group 4 x (ng+1)
     const size_t rig_bytes = align_up(sizeof(RigDesc) * n);
     const size_t scr_bytes = align_up(sizeof(uint64_t*) * n);
     const size_t bs_bytes = align_up(sizeof(int) * (n + 1) * kNumStages);
     size_t gbs_ints = 0;
-    for (auto& G : groups) gbs_ints += 3 * (size_t)(G.i1 - G.i0 + 1);
+    for (auto& G : groups) gbs_ints += 4 * (size_t)(G.i1 - G.i0 + 1);
     const size_t gbs_bytes = align_up(sizeof(int) * gbs_ints);
     std::vector<char> host(rig_bytes + scr_bytes + bs_bytes + gbs_bytes, 0);
     std::memcpy(host.data(), p.host_rigs.data(), sizeof(RigDesc) * n);
@@ -348,17 +352,15 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
     for (auto& G : groups) {
         const int ng = G.i1 - G.i0;
         gofs.push_back(gpos);
-        int tot[3] = {0, 0, 0};
+        int tot[4] = {0, 0, 0, 0};
         for (int k = 0; k < ng; k++) {
-            int nb[3];
-            sgm_group_blocks(p.host_rigs[G.i0 + k], G.KL, nb);
-            for (int f = 0; f < 3; f++) {
+            for (int f = 0; f < 4; f++) {
                 gbs[gpos + f * (ng + 1) + k] = tot[f];
-                tot[f] += nb[f];
+                tot[f] += sgm_family_blocks(p.host_rigs[G.i0 + k], G.KL[f], f);
             }
         }
-        for (int f = 0; f < 3; f++) gbs[gpos + f * (ng + 1) + ng] = tot[f];
-        gpos += 3 * (size_t)(ng + 1);
+        for (int f = 0; f < 4; f++) gbs[gpos + f * (ng + 1) + ng] = tot[f];
+        gpos += 4 * (size_t)(ng + 1);
     }
     if (c->pool.get(host.size(), p.table) != hipSuccess) {
         for (auto& w : p.workspaces) c->pool.put(w);
@@ -381,12 +383,12 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         SgmGroup& sg = p.ls.groups[gi];
         sg.rigs = p.ls.rigs + G.i0;
         sg.nrig = ng;
-        for (int f = 0; f < 3; f++) {
+        for (int f = 0; f < 4; f++) {
             sg.bs[f] = dg + gofs[gi] + f * (ng + 1);
             sg.blocks[f] = gbs[gofs[gi] + f * (ng + 1) + ng];
+            sg.KL[f] = G.KL[f];
+            sg.CH[f] = G.CH[f];
         }
-        sg.KL = G.KL;
-        sg.CH = G.CH;
     }
     return SISTER_OK;
 }
