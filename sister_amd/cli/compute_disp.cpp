@@ -9,8 +9,9 @@
 // into the folder
 //   disp_multiview.png, disp_horizontal.png, disp_vertical.png   16-bit, the
 //       returned maps (255*d, hpp:111-118)
-//   disp_*_vis.png   8-bit min-max normalised views (cv::normalize NORM_MINMAX,
-//       :38-40; the MAGMA colour map of :41-43 is not applied)
+//   disp_*_color.png   min-max normalised to 8 bits and colour-mapped with
+//       MAGMA (:38-43)
+//   blended.png   0.1 * center + 0.9 * coloured multiview map (:46-48)
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -19,17 +20,8 @@
 #include <string>
 
 #include "../../include/sister/SisterMultiviewDisparities.hpp"
+#include "colorize.hpp"
 #include "png_io.hpp"
-
-static void write_vis(const std::string& path, const sister::Mat16& m)
-{
-    uint16_t lo = 65535, hi = 0;
-    for (uint16_t v : m.data) { lo = std::min(lo, v); hi = std::max(hi, v); }
-    std::vector<uint16_t> n(m.data.size());
-    const double s = hi > lo ? 255.0 / (hi - lo) : 0.0;
-    for (size_t i = 0; i < n.size(); i++) n[i] = (uint16_t)(int)((m.data[i] - lo) * s + 0.5);
-    pngio::write_png(path, n.data(), m.cols, m.rows, 8);
-}
 
 int main(int argc, char** argv)
 {
@@ -57,9 +49,18 @@ int main(int argc, char** argv)
         pngio::write_png(folder + "disp_multiview.png", result_multiview.data.data(), result_multiview.cols, result_multiview.rows, 16);
         pngio::write_png(folder + "disp_horizontal.png", result_horizontal.data.data(), result_horizontal.cols, result_horizontal.rows, 16);
         pngio::write_png(folder + "disp_vertical.png", result_vertical.data.data(), result_vertical.cols, result_vertical.rows, 16);
-        write_vis(folder + "disp_multiview_vis.png", result_multiview);
-        write_vis(folder + "disp_horizontal_vis.png", result_horizontal);
-        write_vis(folder + "disp_vertical_vis.png", result_vertical);
+
+        // colorize (:37-43) and blend the multiview map over the center view (:45-48)
+        const sister::Mat8 color_h = colorize::apply_magma(colorize::normalize_minmax(result_horizontal));
+        const sister::Mat8 color_v = colorize::apply_magma(colorize::normalize_minmax(result_vertical));
+        const sister::Mat8 color_m = colorize::apply_magma(colorize::normalize_minmax(result_multiview));
+        const float alpha_blending_rgb = 0.1f;
+        const sister::Mat8 blended =
+            colorize::add_weighted(colorize::to_bgr(center), alpha_blending_rgb, color_m, 1.0f - alpha_blending_rgb, 0.0f);
+        pngio::write_png_bgr(folder + "disp_horizontal_color.png", color_h);
+        pngio::write_png_bgr(folder + "disp_vertical_color.png", color_v);
+        pngio::write_png_bgr(folder + "disp_multiview_color.png", color_m);
+        pngio::write_png_bgr(folder + "blended.png", blended);
     } catch (const std::exception& e) {
         std::cerr << "compute_disp: " << e.what() << std::endl;
         return 1;

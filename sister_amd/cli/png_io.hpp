@@ -149,6 +149,27 @@ inline void chunk(std::vector<uint8_t>& out, const char* type, const std::vector
     put32(out, (uint32_t)crc32(0, td.data(), td.size()));
 }
 
+// PNG from filtered-free scanlines (filter byte 0 already in `raw`)
+inline void write_png_raw(const std::string& path, const std::vector<uint8_t>& raw, int W, int H, int depth,
+                          int color_type)
+{
+    uLongf n = compressBound(raw.size());
+    std::vector<uint8_t> z(n);
+    if (compress2(z.data(), &n, raw.data(), raw.size(), 6) != Z_OK) throw std::runtime_error("deflate failed");
+    z.resize(n);
+    std::vector<uint8_t> out = {137, 80, 78, 71, 13, 10, 26, 10};
+    std::vector<uint8_t> ih;
+    put32(ih, W);
+    put32(ih, H);
+    ih.push_back((uint8_t)depth); ih.push_back((uint8_t)color_type); ih.push_back(0); ih.push_back(0); ih.push_back(0);
+    chunk(out, "IHDR", ih);
+    chunk(out, "IDAT", z);
+    chunk(out, "IEND", {});
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot write " + path);
+    f.write((const char*)out.data(), out.size());
+}
+
 // gray PNG, 8 or 16 bits (samples given as 16-bit; 8-bit keeps the low byte)
 inline void write_png(const std::string& path, const uint16_t* px, int W, int H, int depth)
 {
@@ -163,21 +184,25 @@ inline void write_png(const std::string& path, const uint16_t* px, int W, int H,
             raw.push_back((uint8_t)v);
         }
     }
-    uLongf n = compressBound(raw.size());
-    std::vector<uint8_t> z(n);
-    if (compress2(z.data(), &n, raw.data(), raw.size(), 6) != Z_OK) throw std::runtime_error("deflate failed");
-    z.resize(n);
-    std::vector<uint8_t> out = {137, 80, 78, 71, 13, 10, 26, 10};
-    std::vector<uint8_t> ih;
-    put32(ih, W);
-    put32(ih, H);
-    ih.push_back((uint8_t)depth); ih.push_back(0); ih.push_back(0); ih.push_back(0); ih.push_back(0);
-    chunk(out, "IHDR", ih);
-    chunk(out, "IDAT", z);
-    chunk(out, "IEND", {});
-    std::ofstream f(path, std::ios::binary);
-    if (!f) throw std::runtime_error("cannot write " + path);
-    f.write((const char*)out.data(), out.size());
+    write_png_raw(path, raw, W, H, depth, 0);
+}
+
+// 8-bit colour PNG of a BGR image (stored as RGB)
+inline void write_png_bgr(const std::string& path, const sister::Mat8& img)
+{
+    if (img.channels != 3) throw std::runtime_error("write_png_bgr: expected 3 channels");
+    std::vector<uint8_t> raw;
+    raw.reserve((size_t)img.rows * (img.cols * 3 + 1));
+    for (int y = 0; y < img.rows; y++) {
+        raw.push_back(0);
+        const uint8_t* r = img.ptr(y);
+        for (int x = 0; x < img.cols; x++) {
+            raw.push_back(r[3 * x + 2]);
+            raw.push_back(r[3 * x + 1]);
+            raw.push_back(r[3 * x]);
+        }
+    }
+    write_png_raw(path, raw, img.cols, img.rows, 8, 2);
 }
 
 }  // namespace pngio

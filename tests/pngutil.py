@@ -64,3 +64,23 @@ def read_png16(path):
         row = np.frombuffer(raw[y * stride + 1:(y + 1) * stride], np.uint8)
         out[y] = (row[0::2].astype(np.uint16) << 8 | row[1::2]) if bps == 2 else row
     return out
+
+
+def read_png_rgb8(path):
+    """8-bit RGB (colour type 2), filter type 0 only (what compute_disp writes)."""
+    d = open(path, "rb").read()
+    p = 8
+    idat = b""
+    while p < len(d):
+        n = struct.unpack(">I", d[p:p + 4])[0]
+        t = d[p + 4:p + 8]
+        body = d[p + 8:p + 8 + n]
+        if t == b"IHDR":
+            W, H, depth, ctype = struct.unpack(">IIBB", body[:10])
+            assert depth == 8 and ctype == 2
+        elif t == b"IDAT":
+            idat += body
+        p += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(H, 3 * W + 1)
+    assert np.all(raw[:, 0] == 0)
+    return raw[:, 1:].reshape(H, W, 3).copy()

@@ -42,6 +42,53 @@ def test_png_reader_rgb_to_gray(png_check, tmp_path):
     assert np.array_equal(_decode(png_check, p), sister_amd.bgr_to_gray(bgr).astype(np.int64))
 
 
+@pytest.fixture(scope="module")
+def colorize_check(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("bin") / "colorize_check")
+    subprocess.run(["g++", "-O1", "-std=c++17", os.path.join(ROOT, "tests", "cli", "colorize_check.cpp"),
+                    "-I", os.path.join(ROOT, "include"), "-o", exe], check=True)
+    return exe
+
+
+def _run(exe, *args):
+    out = subprocess.run([exe, *map(str, args)], check=True, capture_output=True, text=True).stdout
+    return np.array(out.split(), np.int64)
+
+
+def test_normalize_minmax(colorize_check):
+    # cv::normalize(NORM_MINMAX, CV_8U): ends map to 0 and 255, halves round to even
+    assert list(_run(colorize_check, "norm", 0, 255, 510, 1020)) == [0, 64, 128, 255]
+    assert list(_run(colorize_check, "norm", 510, 765, 1020)) == [0, 128, 255]   # 127.5 -> 128
+    assert list(_run(colorize_check, "norm", 7, 7, 7)) == [0, 0, 0]             # empty range: scale 0
+    v = np.random.default_rng(5).integers(0, 48, size=40) * 255
+    n = _run(colorize_check, "norm", *v)
+    assert n.min() == 0 and n.max() == 255
+    order = np.argsort(v, kind="stable")
+    assert np.all(np.diff(n[order]) >= 0)                                         # monotone
+    assert np.all(np.abs(n - (v - v.min()) * 255.0 / (v.max() - v.min())) <= 0.5)
+
+
+def test_magma_lut(colorize_check):
+    lut = _run(colorize_check, "lut").reshape(256, 3)[:, ::-1]   # BGR -> RGB
+    # matplotlib's magma at 0, 127/255 and 1 is about (0.0015, 0.0005, 0.0139),
+    # (0.713, 0.214, 0.476) and (0.987, 0.991, 0.750); the polynomial fit stays
+    # within 6/255 of those points (parity with OpenCV's own table is unpinned)
+    for i, rgb in ((0, (0.001462, 0.000466, 0.013866)), (127, (0.712803, 0.213521, 0.475780)),
+                   (255, (0.987053, 0.991438, 0.749504))):
+        assert np.all(np.abs(lut[i] - np.round(np.array(rgb) * 255)) <= 6), (i, lut[i])
+    lum = lut @ np.array([0.299, 0.587, 0.114])
+    assert np.all(np.diff(lum) >= -0.5)      # lightness rises along the map
+    assert lum[255] - lum[0] > 200
+
+
+def test_blend_weights(colorize_check):
+    # addWeighted(center, 0.1, colour, 0.9, 0) in float, rounded half to even
+    assert list(_run(colorize_check, "blend", 255, 0)) == [26, 26, 26]       # 25.5 -> 26
+    assert list(_run(colorize_check, "blend", 0, 255)) == [230, 230, 230]    # 255 * 0.9f rounds to 229.5
+    assert list(_run(colorize_check, "blend", 100, 200)) == [190, 190, 190]
+    assert list(_run(colorize_check, "blend", 0, 0)) == [0, 0, 0]
+
+
 @pytest.mark.gpu
 def test_compute_disp_cli_matches_oracle(oracle_mod, tmp_path):
     cli = os.path.join(ROOT, "sister_amd", "compute_disp")
@@ -58,3 +105,10 @@ def test_compute_disp_cli_matches_oracle(oracle_mod, tmp_path):
     for name in ("multiview", "horizontal", "vertical"):
         got = pngutil.read_png16(str(tmp_path / f"disp_{name}.png"))
         assert np.array_equal(got, ref[name]), name
+        col = pngutil.read_png_rgb8(str(tmp_path / f"disp_{name}_color.png"))
+        assert col.shape == (H, W, 3)
+    # the blend is 0.1 * center + 0.9 * the coloured multiview map
+    col = pngutil.read_png_rgb8(str(tmp_path / "disp_multiview_color.png")).astype(np.float64)
+    blend = pngutil.read_png_rgb8(str(tmp_path / "blended.png")).astype(np.float64)
+    center = rig["views"][0].astype(np.float64)[..., None]
+    assert np.all(np.abs(blend - (0.1 * center + 0.9 * col)) <= 0.5 + 1e-3)
