@@ -10,21 +10,25 @@ enum Stage {
     kStageWta,
     kStageMedianPrep,
     kStageMask,
-    kStageMsum,     // cost sum, frames 0/180 (row runs)
-    kStageMsumV,    // cost sum, frames 90/270 (column runs)
     kNumStages
 };
 
-// SGM family launches run per group of rigs with the same (CH, KL): CH
-// chunks of 16 disparities per lane, KL lanes per chain.
-struct SgmGroup {
+// Launches whose code depends on the disparity count (cost sum, SGM
+// families) run per group of rigs with the same D.
+enum GroupKernel {
+    kGRows = 0, kGCols, kGDiag, kGAnti,   // SGM families
+    kGMsumRows, kGMsumCols,               // cost sum: frames 0/180 (row runs), 90/270 (column runs)
+    kNumGroupKernels
+};
+struct DGroup {
     const RigDesc* rigs;     // device pointer to the group's first rig
     int nrig;
-    const int* bs[4];        // block starts per family: rows, columns, diagonals, anti-diagonals
-    int blocks[4];
-    int CH[4], KL[4];        // chunks per lane, lanes per chain, per family
+    int D, Dp;
+    const int* bs[kNumGroupKernels];   // block starts per kernel, nrig+1 entries each
+    int blocks[kNumGroupKernels];
+    int CH[4], KL[4];        // SGM: chunks of 16 disparities per lane, lanes per chain, per family
 };
-constexpr int kMaxSgmGroups = 16;
+constexpr int kMaxDGroups = 16;
 
 // Everything one launch of a stage needs for a set of rigs (device pointers).
 struct LaunchSet {
@@ -36,7 +40,7 @@ struct LaunchSet {
     int maxD;
     int maxCols;                    // max frame width over the rigs (median LDS)
     int ngroups;
-    SgmGroup groups[kMaxSgmGroups];
+    DGroup groups[kMaxDGroups];
 };
 
 // individual kernels, for per-kernel event timing
@@ -57,6 +61,7 @@ void stage_blocks(const RigDesc& g, int maxD, int out[kNumStages]);
 int sgm_pick_kl(long cols_chains, int fam, int D);
 int sgm_pick_ch(int D, int KL);
 int sgm_family_blocks(const RigDesc& g, int KL, int fam);
+int msum_blocks(const RigDesc& g, int side);
 void launch_census(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t);

@@ -22,6 +22,20 @@
 
 namespace sister {
 
+// Device buffers reached through the rig table are generic pointers; casting
+// them to the global address space lets the compiler use global_load/store
+// with a scalar base (flat accesses need 64-bit per-lane addresses and also
+// hold the LDS counter).
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> G(T* p) { return (gptr<T>)p; }
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 LD16(const void* p)
+{
+    const u32x4 v = *(gptr<const u32x4>)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void ST16(void* p, uint4 v) { *(gptr<u32x4>)p = u32x4{v.x, v.y, v.z, v.w}; }
+
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
@@ -352,17 +366,20 @@ __global__ __launch_bounds__(256) void k_mask(const RigDesc* __restrict__ rigs, 
 // popcount (<= 63) and M <= 252 fits a byte.
 // Each frame's partners of a pixel lie along its frame row, which is an
 // image row for frames 0/180 and an image column for frames 90/270, so two
-// kernels share the work: k_msum_rows (frames 0 and 180, one block = a run of
-// TX pixels of one row) writes M, k_msum_cols (frames 90 and 270, one block =
-// a run of TY pixels of one column) adds to it.  Each block stages the partner
-// window of its run (TX + D - 1 codes per frame) in LDS once; lanes sweep
-// (pixel, d) with consecutive lanes on consecutive d: conflict-free 8-byte
-// LDS reads and coalesced byte stores.  Cell stride Dp, zero past D.
+// launches share the work: k_msum_run<0> (frames 0 and 180, one block = a run
+// of kRun pixels of one row) writes M, k_msum_run<1> (frames 90 and 270, one
+// block = a run of one column) adds to it.  Each block stages the partner
+// window of its run (kRun + D - 1 codes per frame) in LDS once.  A wave takes
+// one pixel at a time, lane q the disparities q + 64 j: conflict-free 8-byte
+// LDS reads at immediate offsets and coalesced 64-byte stores per group, with
+// no per-disparity branches.  Cell stride Dp, zero past D.
 // ---------------------------------------------------------------------------
 constexpr int kRun = 64;    // pixels per block
+constexpr int kWinPad = 8;  // window entries before each partner window
 
-// side = 0: frames 0/180 along row y; side = 1: frames 90/270 along column x
-template <int SIDE>
+// SIDE = 0: frames 0/180 along row y; SIDE = 1: frames 90/270 along column x.
+// J = ceil(Dp / 64) groups of 64 disparities; every rig of the launch has this Dp.
+template <int SIDE, int J>
 __global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ rigs, int nrig,
                                                   const int* __restrict__ bs, int fuse)
 {
@@ -378,8 +395,10 @@ __global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ ri
     const int line = lb / runs, p0 = (lb % runs) * kRun;
     const int len = SIDE == 0 ? Wp : Hp;        // pixels along the line
     const int SPAN = kRun + D - 1;
-    uint64_t* LA = lds64;                       // partners of frame FA
-    uint64_t* LB = LA + SPAN;                   // partners of frame FB
+    // lanes of the last group past D read up to Dp - D <= 8 entries before
+    // their window; those terms are masked
+    uint64_t* LA = lds64 + kWinPad;             // partners of frame FA
+    uint64_t* LB = LA + SPAN + kWinPad;         // partners of frame FB
     uint64_t* LCa = LB + SPAN;                  // centers
     uint64_t* LCb = LCa + kRun;
     uint8_t* LM = (uint8_t*)(LCb + kRun);
@@ -393,18 +412,16 @@ __global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ ri
     const uint64_t* NB = g.code[2 * FB + 1] + (size_t)frow * fcols;
     const uint64_t* CA = g.code[2 * FA] + (size_t)frow * fcols;
     const uint64_t* CB = g.code[2 * FB] + (size_t)frow * fcols;
-    // frame A columns run with the line coordinate (x for F0, y for F3 ...)
-    // for SIDE 0: frame A = F0 (col = x), frame B = F2 (col = W'-1-x)
-    // for SIDE 1: frame A = F1 (col = H'-1-y), frame B = F3 (col = y)
     // window of frame X: the frame columns [lo, lo + SPAN) covering every
-    // partner col - d of the run
+    // partner col - d of the run (frame A columns run with the line
+    // coordinate for F0 and F3, against it for F1 and F2)
     int loA, loB;
     if (SIDE == 0) { loA = p0 - (D - 1); loB = Wp - p0 - kRun - (D - 1); }
     else { loA = Hp - p0 - kRun - (D - 1); loB = p0 - (D - 1); }
-    for (int k = tid; k < SPAN; k += 256) {
+    for (int k = tid - kWinPad; k < SPAN; k += 256) {
         const int ca = loA + k, cb = loB + k;
-        LA[k] = (useA && ca >= 0 && ca < fcols) ? NA[ca] : 0;
-        LB[k] = (useB && cb >= 0 && cb < fcols) ? NB[cb] : 0;
+        LA[k] = (useA && k >= 0 && ca >= 0 && ca < fcols) ? G(NA)[ca] : 0;
+        LB[k] = (useB && k >= 0 && cb >= 0 && cb < fcols) ? G(NB)[cb] : 0;
     }
     for (int i = tid; i < kRun; i += 256) {
         const int pos = p0 + i;                 // x (SIDE 0) or y (SIDE 1)
@@ -415,52 +432,55 @@ __global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ ri
             const int colB = SIDE == 0 ? Wp - 1 - pos : pos;
             const int y = SIDE == 0 ? line : pos, x = SIDE == 0 ? pos : line;
             const size_t q = (size_t)y * Wp + x;
-            if (useA) a = CA[colA];
-            if (useB) b = CB[colB];
+            if (useA) a = G(CA)[colA];
+            if (useB) b = G(CB)[colB];
             // a set mask implies frame column >= D; the check keeps indices in range
-            if (useA && g.mask[FA][q] && colA >= D) mb |= 1;
-            if (useB && g.mask[FB][q] && colB >= D) mb |= 2;
+            if (useA && G(g.mask[FA])[q] && colA >= D) mb |= 1;
+            if (useB && G(g.mask[FB])[q] && colB >= D) mb |= 2;
         }
         LCa[i] = a;
         LCb[i] = b;
         LM[i] = mb;
     }
     __syncthreads();
-    const int QPP = Dp / 4;                     // lanes per pixel (d = q + j*QPP)
-    const int ppass = 256 / QPP;
-    const int slot = tid / QPP, q = tid % QPP;
-    if (slot >= ppass) return;
-    for (int base = 0; base < kRun; base += ppass) {
-        const int i = base + slot;
-        if (i >= kRun) break;
+    // one wave per pixel; lane q owns d = q + 64 j.  In the last group, lanes
+    // at or past Dp read as lane Dp - 1 - 64 (J - 1) (and store nothing),
+    // lanes at or past D add nothing.
+    const int wave = tid >> 6, q = tid & 63;
+    const int qlast = min(q, Dp - 1 - 64 * (J - 1));
+    const bool store_last = q + 64 * (J - 1) < Dp;
+    const uint32_t mask_last = (q + 64 * (J - 1) < D) ? 0xFFFFFFFFu : 0u;
+    auto add_frame = [&](const uint64_t* L, uint64_t cc, int base, unsigned (&acc)[J]) {
+        uint64_t t[J];
+        const uint64_t* w = L + base - q;       // w[-64 j] = window[base - d]
+#pragma unroll
+        for (int j = 0; j < J - 1; j++) t[j] = w[-64 * j];
+        t[J - 1] = L[base - qlast - 64 * (J - 1)];
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const unsigned v = (unsigned)__popcll(cc ^ t[j]);
+            acc[j] += (j == J - 1) ? (v & mask_last) : v;
+        }
+    };
+    for (int i = wave; i < kRun; i += 4) {
         const int pos = p0 + i;
         if (pos >= len) break;
         const int y = SIDE == 0 ? line : pos, x = SIDE == 0 ? pos : line;
-        uint8_t* o = g.msum + ((size_t)y * Wp + x) * Dp;
-        unsigned acc[4] = {0, 0, 0, 0};
+        gptr<uint8_t> o = G(g.msum) + ((size_t)y * Wp + x) * Dp;
+        unsigned acc[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) acc[j] = 0;
         if (SIDE == 1 && (fuse & 5)) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc[j] = o[q + j * QPP];
+            for (int j = 0; j < J - 1; j++) acc[j] = o[q + 64 * j];
+            if (store_last) acc[J - 1] = o[q + 64 * (J - 1)];
         }
-        const uint8_t mb = LM[i];
-        // partner of run pixel i at disparity d: frame column col - d, whose
-        // window index is col - d - lo
-        if (mb & 1) {
-            const uint64_t cc = LCa[i];
-            const int colA = SIDE == 0 ? pos : Hp - 1 - pos;
-            const uint64_t* w = LA + (colA - loA);
+        const int mb = __builtin_amdgcn_readfirstlane(LM[i]);   // one pixel per wave
+        if (mb & 1) add_frame(LA, LCa[i], (SIDE == 0 ? pos : Hp - 1 - pos) - loA, acc);
+        if (mb & 2) add_frame(LB, LCb[i], (SIDE == 0 ? Wp - 1 - pos : pos) - loB, acc);
 #pragma unroll
-            for (int j = 0; j < 4; j++) { const int d = q + j * QPP; if (d < D) acc[j] += __popcll(cc ^ w[-d]); }
-        }
-        if (mb & 2) {
-            const uint64_t cc = LCb[i];
-            const int colB = SIDE == 0 ? Wp - 1 - pos : pos;
-            const uint64_t* w = LB + (colB - loB);
-#pragma unroll
-            for (int j = 0; j < 4; j++) { const int d = q + j * QPP; if (d < D) acc[j] += __popcll(cc ^ w[-d]); }
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) o[q + j * QPP] = (uint8_t)acc[j];
+        for (int j = 0; j < J - 1; j++) o[q + 64 * j] = (uint8_t)acc[j];
+        if (store_last) o[q + 64 * (J - 1)] = (uint8_t)acc[J - 1];
     }
 }
 
@@ -735,13 +755,13 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
             static_for<0, CH>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 const bool ok = act && cg.lid * CH + q < cg.nq;
-                b.m[q] = ok ? *(const uint4*)(g.msum + off + 16 * q) : make_uint4(0, 0, 0, 0);
-                if (need_p) b.p[q] = ok ? *(const uint4*)(part + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                b.m[q] = ok ? LD16(g.msum + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                if (need_p) b.p[q] = ok ? LD16(part + off + 16 * q) : make_uint4(0, 0, 0, 0);
                 if (need_o) {
-                    b.o[q] = ok ? *(const uint4*)(g.part[0] + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                    b.o[q] = ok ? LD16(g.part[0] + off + 16 * q) : make_uint4(0, 0, 0, 0);
                     if (NO == 3) {
-                        b.o[CH + q] = ok ? *(const uint4*)(g.part[1] + off + 16 * q) : make_uint4(0, 0, 0, 0);
-                        b.o[2 * CH + q] = ok ? *(const uint4*)(g.part[2] + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                        b.o[CH + q] = ok ? LD16(g.part[1] + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                        b.o[2 * CH + q] = ok ? LD16(g.part[2] + off + 16 * q) : make_uint4(0, 0, 0, 0);
                     }
                 }
             });
@@ -787,11 +807,11 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                         uint4 bb = make_uint4(st.L[8 * q + 4], st.L[8 * q + 5], st.L[8 * q + 6], st.L[8 * q + 7]);
                         if (cg.half_tail && cg.lid * CH + q == cg.nq - 1) bb = make_uint4(0, 0, 0, 0);
                         if (dir == 1) {
-                            a = add_u16x8(a, *(const uint4*)e);
-                            bb = add_u16x8(bb, *(const uint4*)(e + 8));
+                            a = add_u16x8(a, LD16(e));
+                            bb = add_u16x8(bb, LD16(e + 8));
                         }
-                        *(uint4*)e = a;
-                        *(uint4*)(e + 8) = bb;
+                        ST16(e, a);
+                        ST16(e + 8, bb);
                     } else {
                         if (dir == 1) {
                             uint32_t pf[8];
@@ -799,7 +819,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 #pragma unroll
                             for (int i = 0; i < 8; i++) tp[q][i] = W2(U2(tp[q][i]) + U2(pf[i]));
                         }
-                        *(uint4*)(part + off + 16 * q) = pack16(tp[q]);
+                        ST16(part + off + 16 * q, pack16(tp[q]));
                     }
                 });
                 return;
@@ -825,8 +845,8 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                             uint4 acc = make_uint4(0, 0, 0, 0);
 #pragma unroll
                             for (int fm = 0; fm < 4; fm++)
-                                acc = add_u16x8(acc, *(const uint4*)(g.edge + (size_t)(fm * 2 + 1) * Wp * Dp + eo + 8 * h));
-                            *(uint4*)(g.sum + ((size_t)y * Wp) * Dp + eo + 8 * h) = acc;
+                                acc = add_u16x8(acc, LD16(g.edge + (size_t)(fm * 2 + 1) * Wp * Dp + eo + 8 * h));
+                            ST16(g.sum + ((size_t)y * Wp) * Dp + eo + 8 * h, acc);
                         }
                     });
                 }
@@ -848,24 +868,24 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                     uint4 a = make_uint4(st.L[8 * q + 0], st.L[8 * q + 1], st.L[8 * q + 2], st.L[8 * q + 3]);
                     uint4 bb = make_uint4(st.L[8 * q + 4], st.L[8 * q + 5], st.L[8 * q + 6], st.L[8 * q + 7]);
                     if (cg.half_tail && qg == cg.nq - 1) bb = make_uint4(0, 0, 0, 0);
-                    *(uint4*)e = a;
-                    *(uint4*)(e + 8) = bb;
+                    ST16(e, a);
+                    ST16(e + 8, bb);
                     if (final_pass && keep_sum && g.sum && y == 0) {
                         // S of row 0: the other families' edge rows + this sweep
                         const size_t eo = (size_t)x * Dp + qg * 16;
                         uint4 s0 = a, s1 = bb;
 #pragma unroll
                         for (int fm = 0; fm < 3; fm++) {
-                            s0 = add_u16x8(s0, *(const uint4*)(g.edge + (size_t)(fm * 2) * Wp * Dp + eo));
-                            s1 = add_u16x8(s1, *(const uint4*)(g.edge + (size_t)(fm * 2) * Wp * Dp + eo + 8));
+                            s0 = add_u16x8(s0, LD16(g.edge + (size_t)(fm * 2) * Wp * Dp + eo));
+                            s1 = add_u16x8(s1, LD16(g.edge + (size_t)(fm * 2) * Wp * Dp + eo + 8));
                         }
-                        *(uint4*)(g.sum + eo) = s0;
-                        *(uint4*)(g.sum + eo + 8) = s1;
+                        ST16(g.sum + eo, s0);
+                        ST16(g.sum + eo + 8, s1);
                     }
                     return;
                 }
                 if (dir == 0) {
-                    *(uint4*)(part + off + 16 * q) = pack16(tp[q]);
+                    ST16(part + off + 16 * q, pack16(tp[q]));
                     return;
                 }
                 uint32_t pf[8];
@@ -873,7 +893,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 #pragma unroll
                 for (int i = 0; i < 8; i++) tp[q][i] = W2(U2(tp[q][i]) + U2(pf[i]));
                 if (!final_pass) {
-                    *(uint4*)(part + off + 16 * q) = pack16(tp[q]);
+                    ST16(part + off + 16 * q, pack16(tp[q]));
                     return;
                 }
                 if (!(crop || keep_sum)) return;
@@ -889,8 +909,8 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                 }
                 if (keep_sum && g.sum) {
                     uint16_t* sc = g.sum + (off - boff) + qg * 16;
-                    *(uint4*)sc = make_uint4(S[0], S[1], S[2], S[3]);
-                    *(uint4*)(sc + 8) = make_uint4(S[4], S[5], S[6], S[7]);
+                    ST16(sc, make_uint4(S[0], S[1], S[2], S[3]));
+                    ST16(sc + 8, make_uint4(S[4], S[5], S[6], S[7]));
                 }
                 if (!crop) return;
 #pragma unroll
@@ -910,8 +930,8 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                 if (crop && cg.lid == 0) {
                     const uint32_t best = key & 511u;
                     const size_t o = (size_t)oy * g.W + ox;
-                    if (outm) outm[o] = (uint16_t)min(best * 255u, 65535u);
-                    if (mode == 0 && g.disp) g.disp[o] = (uint16_t)best;
+                    if (outm) G(outm)[o] = (uint16_t)min(best * 255u, 65535u);
+                    if (mode == 0 && g.disp) G(g.disp)[o] = (uint16_t)best;
                     if (want_sub) {
                         float delta = 0.0f;
                         if (best > 0 && (int)best < D - 1) {
@@ -920,7 +940,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                             const int den = sm - 2 * s0 + sp;
                             if (den > 0) delta = (float)(sm - sp) / (2.0f * (float)den);
                         }
-                        g.subpix[o] = (float)best + delta;
+                        G(g.subpix)[o] = (float)best + delta;
                     }
                 }
                 if (want_sub) __syncthreads();
@@ -1046,28 +1066,49 @@ void launch_mask(const LaunchSet& ls, int fmask, hipStream_t st, KernelTimer* t)
     TE(kKMask);
 }
 
-size_t msum_lds(int maxD)
+size_t msum_lds(int D)
 {
-    return (size_t)2 * (kRun + maxD - 1) * 8 + (size_t)2 * kRun * 8 + kRun;
+    return (size_t)(2 * kWinPad + 2 * (kRun + D - 1) + 2 * kRun) * 8 + kRun;
+}
+
+int msum_blocks(const RigDesc& g, int side)
+{
+    return side == 0 ? g.Hp * ((g.Wp + kRun - 1) / kRun) : g.Wp * ((g.Hp + kRun - 1) / kRun);
+}
+
+template <int SIDE>
+static void launch_msum_side(const DGroup& G, int fuse, hipStream_t st)
+{
+    const size_t lds = msum_lds(G.D);
+    const int k = kGMsumRows + SIDE;
+    const dim3 grid(G.blocks[k]), block(256);
+    switch ((G.Dp + 63) / 64) {
+    case 1: hipLaunchKernelGGL((k_msum_run<SIDE, 1>), grid, block, lds, st, G.rigs, G.nrig, G.bs[k], fuse); break;
+    case 2: hipLaunchKernelGGL((k_msum_run<SIDE, 2>), grid, block, lds, st, G.rigs, G.nrig, G.bs[k], fuse); break;
+    case 3: hipLaunchKernelGGL((k_msum_run<SIDE, 3>), grid, block, lds, st, G.rigs, G.nrig, G.bs[k], fuse); break;
+    case 4: hipLaunchKernelGGL((k_msum_run<SIDE, 4>), grid, block, lds, st, G.rigs, G.nrig, G.bs[k], fuse); break;
+    case 5: hipLaunchKernelGGL((k_msum_run<SIDE, 5>), grid, block, lds, st, G.rigs, G.nrig, G.bs[k], fuse); break;
+    case 6: hipLaunchKernelGGL((k_msum_run<SIDE, 6>), grid, block, lds, st, G.rigs, G.nrig, G.bs[k], fuse); break;
+    case 7: hipLaunchKernelGGL((k_msum_run<SIDE, 7>), grid, block, lds, st, G.rigs, G.nrig, G.bs[k], fuse); break;
+    default: hipLaunchKernelGGL((k_msum_run<SIDE, 8>), grid, block, lds, st, G.rigs, G.nrig, G.bs[k], fuse); break;
+    }
 }
 
 void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t)
 {
-    const size_t lds = msum_lds(ls.maxD);
     TB(kKMsum);
-    if (fuse & 5)   // frames 0/180 write M (and zero it when they are the only ones)
-        hipLaunchKernelGGL(k_msum_run<0>, dim3(ls.blocks[kStageMsum]), dim3(256), lds, st, ls.rigs, ls.nrig,
-                           ls.bs[kStageMsum], fuse);
-    if (fuse & 10)  // frames 90/270 add (or write when 0/180 are unused)
-        hipLaunchKernelGGL(k_msum_run<1>, dim3(ls.blocks[kStageMsumV]), dim3(256), lds, st, ls.rigs, ls.nrig,
-                           ls.bs[kStageMsumV], fuse);
+    for (int gi = 0; gi < ls.ngroups; gi++) {
+        const DGroup& G = ls.groups[gi];
+        if (fuse & 5) launch_msum_side<0>(G, fuse, st);    // frames 0/180 write M (zero when alone)
+        if (fuse & 10) launch_msum_side<1>(G, fuse, st);   // frames 90/270 add (or write when alone)
+    }
     TE(kKMsum);
 }
 
 void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, KernelTimer* t)
 {
     for (int gi = 0; gi < ls.ngroups; gi++) {
-        const SgmGroup& G = ls.groups[gi];
+        const DGroup& G = ls.groups[gi];
         const size_t lds = (size_t)64 * G.CH[3] * 32;
         TB(kKRows);
         SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
@@ -1093,8 +1134,6 @@ void stage_blocks(const RigDesc& g, int maxD, int out[kNumStages])
     out[kStageWta] = 2 * g.Hp * ((g.Wp + 255) / 256) + 2 * g.Wp * ((g.Hp + 255) / 256);
     out[kStageMedianPrep] = (int)(8 * ((npx + 255) / 256));
     out[kStageMask] = (int)(4 * ((npx + 255) / 256));
-    out[kStageMsum] = g.Hp * ((g.Wp + kRun - 1) / kRun);
-    out[kStageMsumV] = g.Wp * ((g.Hp + kRun - 1) / kRun);
 }
 
 }  // namespace sister

@@ -316,17 +316,17 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         groups.push_back(G);
         k = e;
     }
-    if ((int)groups.size() > kMaxSgmGroups) {
+    if ((int)groups.size() > kMaxDGroups) {
         for (auto& w : p.workspaces) c->pool.put(w);
         p.workspaces.clear();
         return fail(c, SISTER_EINVAL, "too many distinct disparity counts in one batch");
     }
-    // table: rigs | scratch ptrs | bs[kNumStages][n+1] | per group 4 x (ng+1)
+    // table: rigs | scratch ptrs | bs[kNumStages][n+1] | per group kNumGroupKernels x (ng+1)
     const size_t rig_bytes = align_up(sizeof(RigDesc) * n);
     const size_t scr_bytes = align_up(sizeof(uint64_t*) * n);
     const size_t bs_bytes = align_up(sizeof(int) * (n + 1) * kNumStages);
     size_t gbs_ints = 0;
-    for (auto& G : groups) gbs_ints += 4 * (size_t)(G.i1 - G.i0 + 1);
+    for (auto& G : groups) gbs_ints += kNumGroupKernels * (size_t)(G.i1 - G.i0 + 1);
     const size_t gbs_bytes = align_up(sizeof(int) * gbs_ints);
     std::vector<char> host(rig_bytes + scr_bytes + bs_bytes + gbs_bytes, 0);
     std::memcpy(host.data(), p.host_rigs.data(), sizeof(RigDesc) * n);
@@ -352,15 +352,16 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
     for (auto& G : groups) {
         const int ng = G.i1 - G.i0;
         gofs.push_back(gpos);
-        int tot[4] = {0, 0, 0, 0};
+        int tot[kNumGroupKernels] = {};
         for (int k = 0; k < ng; k++) {
-            for (int f = 0; f < 4; f++) {
+            const RigDesc& g = p.host_rigs[G.i0 + k];
+            for (int f = 0; f < kNumGroupKernels; f++) {
                 gbs[gpos + f * (ng + 1) + k] = tot[f];
-                tot[f] += sgm_family_blocks(p.host_rigs[G.i0 + k], G.KL[f], f);
+                tot[f] += f < 4 ? sgm_family_blocks(g, G.KL[f], f) : msum_blocks(g, f - kGMsumRows);
             }
         }
-        for (int f = 0; f < 4; f++) gbs[gpos + f * (ng + 1) + ng] = tot[f];
-        gpos += 4 * (size_t)(ng + 1);
+        for (int f = 0; f < kNumGroupKernels; f++) gbs[gpos + f * (ng + 1) + ng] = tot[f];
+        gpos += kNumGroupKernels * (size_t)(ng + 1);
     }
     if (c->pool.get(host.size(), p.table) != hipSuccess) {
         for (auto& w : p.workspaces) c->pool.put(w);
@@ -380,12 +381,16 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
     for (size_t gi = 0; gi < groups.size(); gi++) {
         const GroupHost& G = groups[gi];
         const int ng = G.i1 - G.i0;
-        SgmGroup& sg = p.ls.groups[gi];
+        DGroup& sg = p.ls.groups[gi];
         sg.rigs = p.ls.rigs + G.i0;
         sg.nrig = ng;
-        for (int f = 0; f < 4; f++) {
+        sg.D = p.host_rigs[G.i0].D;
+        sg.Dp = p.host_rigs[G.i0].Dp;
+        for (int f = 0; f < kNumGroupKernels; f++) {
             sg.bs[f] = dg + gofs[gi] + f * (ng + 1);
             sg.blocks[f] = gbs[gofs[gi] + f * (ng + 1) + ng];
+        }
+        for (int f = 0; f < 4; f++) {
             sg.KL[f] = G.KL[f];
             sg.CH[f] = G.CH[f];
         }
