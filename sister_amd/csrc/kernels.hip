@@ -110,53 +110,78 @@ __device__ __forceinline__ unsigned wave_min(unsigned v)
 }
 
 // ---------------------------------------------------------------------------
-// K1 census (census.cpp:38-51), one 16x16 tile of frame pixels per block.
+// K1 census (census.cpp:38-51), one 32x32 tile of frame pixels per block (4
+// pixels per thread, so that staging the tile's window costs less per pixel).
 // Test bit for offset (y,x), y=-3..3 outer, x=-4..4 inner, is
 // I(r+y, c+x) > I(r-y, c-x), shifted in MSB-first; bit 63 carries bit 0 of
 // the previous domain pixel in frame raster order (the accumulator is never
 // reset, census.cpp:42), i.e. the (3,4) test of that pixel.
 // ---------------------------------------------------------------------------
+constexpr int kCT = 32;   // tile side (frame pixels)
+
 __global__ __launch_bounds__(256) void k_census(const RigDesc* __restrict__ rigs, int nrig,
                                                 const int* __restrict__ bs)
 {
-    __shared__ uint8_t tile[22][26];
+    constexpr int TR = kCT + 6, TC = kCT + 9;   // rows r0-3 .., cols c0-5 .. (c0-5: the carry test)
+    __shared__ uint8_t tile[TR][TC];
     const int ri = find_rig(bs, nrig, blockIdx.x);
     const RigDesc& g = rigs[ri];
     const int lb = blockIdx.x - bs[ri];
-    const int T = ((g.Wp + 15) / 16) * ((g.Hp + 15) / 16);
+    const int T = ((g.Wp + kCT - 1) / kCT) * ((g.Hp + kCT - 1) / kCT);
     const int unit = lb / T, t = lb % T;
     const int F = unit >> 1;
     const int R = frame_rows(g, F), C = frame_cols(g, F);
-    const int tilesC = (C + 15) / 16;
-    const int r0 = (t / tilesC) * 16, c0 = (t % tilesC) * 16;
+    const int tilesC = (C + kCT - 1) / kCT;
+    const int r0 = (t / tilesC) * kCT, c0 = (t % tilesC) * kCT;
     const uint8_t* v = g.view[(unit & 1) ? 1 + F : 0];
 
-    for (int i = threadIdx.x; i < 22 * 25; i += 256) {
-        int rr = i / 25, cc = i % 25;
-        int r = r0 - 3 + rr, c = c0 - 5 + cc;
+    // frames 90/270 are transposes: step along frame rows there, so that
+    // consecutive lanes read along an image row
+    const bool odd = F & 1;
+    for (int i = threadIdx.x; i < TR * TC; i += 256) {
+        const int rr = odd ? i % TR : i / TC, cc = odd ? i / TR : i % TC;
+        const int r = r0 - 3 + rr, c = c0 - 5 + cc;
         tile[rr][cc] = (r >= 0 && r < R && c >= 0 && c < C) ? (uint8_t)frame_px(g, v, F, r, c) : 0;
     }
     __syncthreads();
-    const int r = r0 + (threadIdx.x >> 4), c = c0 + (threadIdx.x & 15);
-    if (r >= R || c >= C) return;
-    uint64_t code = 0;
-    if (r >= 3 && r < R - 3 && c >= 4 && c < C - 4) {
-        const int lr = r - r0 + 3, lc = c - c0 + 5;
+    const int c = c0 + (threadIdx.x & 31);
+    if (c >= C) return;
+    uint64_t* out = g.code[unit];
 #pragma unroll
-        for (int y = -3; y <= 3; y++)
+    for (int k = 0; k < kCT / 8; k++) {
+        const int r = r0 + (threadIdx.x >> 5) + 8 * k;
+        if (r >= R) break;
+        uint64_t code = 0;
+        if (r >= 3 && r < R - 3 && c >= 4 && c < C - 4) {
+            const int lr = r - r0 + 3, lc = c - c0 + 5;
+            // test t (t = 0..62, offset (t/9 - 3, t%9 - 4)) lands in bit 62 - t.
+            // Tests t and 62 - t compare the same two pixels in opposite order,
+            // so each pair gives both bits: a > b is the sign of b - a, and the
+            // mirror test's bit the sign of a - b.  Tests 0..30 fill bits 62..32
+            // (hi), the center test 31 is always 0 (bit 31), tests 32..62 fill
+            // bits 30..0 (lo); each bit is shifted in with one alignbit.
+            uint32_t hi = 0, lo = 0;
+            int da[31];
 #pragma unroll
-            for (int x = -4; x <= 4; x++)
-                code = (code << 1) | (uint64_t)(tile[lr + y][lc + x] > tile[lr - y][lc - x]);
-        unsigned carry;
-        if (c > 4)
-            carry = tile[lr + 3][lc + 3] > tile[lr - 3][lc - 5];          // previous pixel (r, c-1)
-        else if (r > 3)
-            carry = frame_px(g, v, F, r + 2, C - 1) > frame_px(g, v, F, r - 4, C - 9);  // (r-1, C-5)
-        else
-            carry = 0;
-        code |= (uint64_t)carry << 63;
+            for (int u = 0; u < 31; u++) {
+                const int y = u / 9 - 3, x = u % 9 - 4;
+                da[u] = (int)tile[lr + y][lc + x] - (int)tile[lr - y][lc - x];
+                hi = __builtin_amdgcn_alignbit(hi, (uint32_t)(-da[u]), 31);
+            }
+#pragma unroll
+            for (int u = 30; u >= 0; u--) lo = __builtin_amdgcn_alignbit(lo, (uint32_t)da[u], 31);
+            code = ((uint64_t)hi << 32) | lo;
+            unsigned carry;
+            if (c > 4)
+                carry = tile[lr + 3][lc + 3] > tile[lr - 3][lc - 5];          // previous pixel (r, c-1)
+            else if (r > 3)
+                carry = frame_px(g, v, F, r + 2, C - 1) > frame_px(g, v, F, r - 4, C - 9);  // (r-1, C-5)
+            else
+                carry = 0;
+            code |= (uint64_t)carry << 63;
+        }
+        G(out)[(size_t)r * C + c] = code;
     }
-    g.code[unit][(size_t)r * C + c] = code;
 }
 
 // ---------------------------------------------------------------------------
@@ -374,7 +399,7 @@ __global__ __launch_bounds__(256) void k_mask(const RigDesc* __restrict__ rigs, 
 // LDS reads at immediate offsets and coalesced 64-byte stores per group, with
 // no per-disparity branches.  Cell stride Dp, zero past D.
 // ---------------------------------------------------------------------------
-constexpr int kRun = 64;    // pixels per block
+constexpr int kRun = 256;   // pixels per block
 constexpr int kWinPad = 8;  // window entries before each partner window
 
 // SIDE = 0: frames 0/180 along row y; SIDE = 1: frames 90/270 along column x.
@@ -1130,7 +1155,7 @@ void stage_blocks(const RigDesc& g, int maxD, int out[kNumStages])
 {
     (void)maxD;
     const long npx = (long)g.Wp * g.Hp;
-    out[kStageCensus] = 8 * ((g.Wp + 15) / 16) * ((g.Hp + 15) / 16);
+    out[kStageCensus] = 8 * ((g.Wp + kCT - 1) / kCT) * ((g.Hp + kCT - 1) / kCT);
     out[kStageWta] = 2 * g.Hp * ((g.Wp + 255) / 256) + 2 * g.Wp * ((g.Hp + 255) / 256);
     out[kStageMedianPrep] = (int)(8 * ((npx + 255) / 256));
     out[kStageMask] = (int)(4 * ((npx + 255) / 256));

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -197,7 +198,13 @@ struct sister_ctx {
     Plan last;                 // plan of the last sister_run_device (kept for debug fetch)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    Plan graph_plan;
+    // a captured batch runs as independent parts on their own streams
+    // (parallel branches of the graph): one part's latency-bound stages (the
+    // serial median, launch tails) overlap another part's work
+    static constexpr int kMaxParts = 8;
+    Plan graph_plan[kMaxParts];
+    hipStream_t side[kMaxParts] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxParts] = {};
     std::mutex mu;             // guards pool / plans between the serving thread and callers
 
     // serving
@@ -453,7 +460,13 @@ int sister_create(const sister_config* cfg, sister_ctx** out)
         delete c;
         return SISTER_EDEVICE;
     }
-    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    bool ok = hipSetDevice(c->device) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 1; ok && k < sister_ctx::kMaxParts; k++)
+        ok = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         delete c;
         return SISTER_EDEVICE;
     }
@@ -476,9 +489,14 @@ void sister_destroy(sister_ctx* c)
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     free_plan(c, c->last);
-    free_plan(c, c->graph_plan);
+    for (auto& gp : c->graph_plan) free_plan(c, gp);
     c->timer.collect();
     c->pool.release_all();
+    (void)hipEventDestroy(c->ev_fork);
+    for (int k = 1; k < sister_ctx::kMaxParts; k++) {
+        (void)hipEventDestroy(c->ev_join[k]);
+        (void)hipStreamDestroy(c->side[k]);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -584,11 +602,45 @@ int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
-    free_plan(c, c->graph_plan);
-    rc = build_plan(c, rigs, n, c->graph_plan);
-    if (rc) return rc;
+    for (auto& gp : c->graph_plan) free_plan(c, gp);
+    // SISTER_GRAPH_PARTS (1..8, default 2) sets how many independent parts a
+    // captured batch is split into (for tuning)
+    static const int want_parts = [] {
+        const char* e = getenv("SISTER_GRAPH_PARTS");
+        const int v = e ? atoi(e) : 2;
+        return v < 1 ? 1 : (v > sister_ctx::kMaxParts ? sister_ctx::kMaxParts : v);
+    }();
+    const int nparts = std::min(want_parts, n);
+    // parts balanced by padded cells (largest rig first to the lightest part)
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    auto cells = [&](int i) {
+        const sister_rig& r = rigs[i];
+        return (double)(r.width + 2 * r.disp_count) * (r.height + 2 * r.disp_count) * r.disp_count;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    std::vector<sister_rig> part[sister_ctx::kMaxParts];
+    double load[sister_ctx::kMaxParts] = {};
+    for (int i : order) {
+        const int h = (int)(std::min_element(load, load + nparts) - load);
+        part[h].push_back(rigs[i]);
+        load[h] += cells(i);
+    }
+    for (int h = 0; h < nparts; h++) {
+        rc = build_plan(c, part[h].data(), (int)part[h].size(), c->graph_plan[h]);
+        if (rc) {
+            for (auto& gp : c->graph_plan) free_plan(c, gp);
+            return rc;
+        }
+    }
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue(c, c->graph_plan, modes, c->stream, nullptr);
+    (void)hipEventRecord(c->ev_fork, c->stream);
+    for (int h = 1; h < nparts; h++) (void)hipStreamWaitEvent(c->side[h], c->ev_fork, 0);
+    for (int h = 0; h < nparts; h++) enqueue(c, c->graph_plan[h], modes, h ? c->side[h] : c->stream, nullptr);
+    for (int h = 1; h < nparts; h++) {
+        (void)hipEventRecord(c->ev_join[h], c->side[h]);
+        (void)hipStreamWaitEvent(c->stream, c->ev_join[h], 0);
+    }
     hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
     if (e != hipSuccess) return fail(c, SISTER_EDEVICE, std::string("capture: ") + hipGetErrorString(e));
     HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
