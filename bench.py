@@ -149,9 +149,12 @@ def run(args):
     ctx.sync()
     torch.cuda.synchronize()
     plain_ms = (time.perf_counter() - t0) * 1e3 / nplain
-    plain_out = [d["outs"]["disp"].cpu().numpy().copy() for d in dev[:min(B, 4)]]
-    r0 = rigs[0]
-    plain_evals = r0["W"] * r0["H"] * r0["D"]
+    plain_evals = sum(rigs[i % B]["W"] * rigs[i % B]["H"] * rigs[i % B]["D"] for i in range(nplain)) / nplain
+    ncmp = min(B, 4)
+    for i in range(nplain, ncmp):          # every compared rig computed alone once
+        ctx.run_device([dev[i]["struct"]], 1)
+    ctx.sync()
+    plain_out = [d["outs"]["disp"].cpu().numpy().copy() for d in dev[:ncmp]]
 
     # ---- batched: all B rigs in one captured plan ----------------------------
     for d in dev:
@@ -170,13 +173,17 @@ def run(args):
     same = all(np.array_equal(dev[i]["outs"]["disp"].cpu().numpy(), plain_out[i]) for i in range(len(plain_out)))
 
     # ---- per-kernel device time: instrumented eager steps of the same batch --
+    # (cost-sum and SGM launches run once per group of equal D: the figures
+    # are per-step totals over a kernel's launches)
+    nrep = max(1, min(args.steps, 3))
     ctx.profile(True)
-    for _ in range(max(1, min(args.steps, 3))):
+    for _ in range(nrep):
         ctx.run_device([d["struct"] for d in dev], 1)
     ctx.sync()
     prof = ctx.profile_read()
     ctx.profile(False)
-    kern = {k: ms / n for k, (ms, n) in prof.items() if n}
+    kern = {k: ms / nrep for k, (ms, n) in prof.items() if n}
+    launches = {k: n / nrep for k, (ms, n) in prof.items() if n}
     dom = max(kern, key=lambda k: kern[k])
     if dom in BYTES_PER_CELL:
         achieved = BYTES_PER_CELL[dom] * cells / (kern[dom] * 1e-3)
@@ -189,7 +196,7 @@ def run(args):
                 traffic = k["fetch_bytes"] + k["write_bytes"]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                    "bytes_per_cell": BYTES_PER_CELL[dom]}
+                    "bytes_per_cell": BYTES_PER_CELL[dom], "launches_per_step": launches[dom]}
     else:
         achieved = OPS_PER_CELL.get(dom, 0) * cells / (kern[dom] * 1e-3)
         roofline = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
@@ -226,6 +233,7 @@ def run(args):
         "pipeline_valu_frac": 97 * cells / (step_ms * 1e-3) / VALU_PEAK,
         "roofline": roofline,
         "kernel_ms": kern,
+        "kernel_launches_per_step": launches,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -17,7 +17,8 @@
  * Images are 8-bit gray, W x H with a row stride in bytes.  W and H must be
  * multiples of 4 and D a multiple of 8 with 8 <= D <= 512 (the reference's own
  * domain is D <= 256: its WTARight_SSE buffer, postprocess.cpp:193; larger D
- * runs here and is flagged by sister_outside_reference_domain()).
+ * runs here and is flagged by sister_outside_reference_domain()), and the
+ * padded sides W + 2D and H + 2D at most 16384.
  * Output maps are W x H uint16 holding min(255 * d, 65535) exactly like the
  * reference's crop-and-scale (hpp:111-118).
  */

@@ -67,7 +67,14 @@ void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_mask(const LaunchSet& ls, int fmask, hipStream_t st, KernelTimer* t);
 void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t);
-// keep_sum: also store the final S of cells inside the crop (debug)
-void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, KernelTimer* t);
+// Streams that let the rows, columns and diagonals families run concurrently
+// (they are independent; the anti-diagonal family, last, reads all three).
+struct FamStreams {
+    hipStream_t side[2];
+    hipEvent_t fork, join[2];
+};
+// keep_sum: also store the final S of cells inside the crop (debug).
+// fs == nullptr runs the four families in sequence on st.
+void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, const FamStreams* fs, KernelTimer* t);
 
 }  // namespace sister

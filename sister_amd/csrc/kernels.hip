@@ -300,7 +300,13 @@ __device__ __forceinline__ unsigned med9_pre(uint64_t s, unsigned a, unsigned b,
     return min(min(s4, max(s3, a)), min(max(s2, b), max(s1, c)));
 }
 
-// One block per (rig, map), 1024 threads; rows in sequence, columns in parallel.
+// One block per (rig, map), 1024 threads; rows in sequence, columns in
+// parallel (thread t owns columns t, t + 1024, ...; frame rows are at most
+// kMedMaxK * 1024 wide).  The order statistics of row r + 1 are loaded while
+// row r is computed.  The last column of a row reads column 0 of the same row,
+// which thread 0 has just computed itself, so one barrier per row suffices.
+constexpr int kMedMaxK = 16;
+
 __global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restrict__ rigs,
                                                         uint64_t* const* __restrict__ scratch)
 {
@@ -318,38 +324,58 @@ __global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restric
     uint16_t* cur = lds16 + C;
     const int tid = threadIdx.x;
     for (int c = tid; c < C; c += 1024) {
-        uint16_t v = in[c];
+        uint16_t v = G(in)[c];
         prev[c] = v;
-        out[c] = v;
+        G(out)[c] = v;
     }
+    // order statistics of a row: columns tid + 1024 k (k < kMedMaxK) below
+    // C - 1, and for thread 0 the last column too
+    uint64_t nx[kMedMaxK], nlast = 0;
+    auto fetch = [&](int r) __attribute__((always_inline)) {
+        const long base = (long)r * C;
+#pragma unroll
+        for (int k = 0; k < kMedMaxK; k++) {
+            const int c = tid + 1024 * k;
+            const long q = base + c;
+            nx[k] = (r < R && c < C - 1 && q <= qhi) ? G(pre)[q] : 0;
+        }
+        if (tid == 0) nlast = (r < R && base + C - 1 <= qhi) ? G(pre)[base + C - 1] : 0;
+    };
+    fetch(1);
     __syncthreads();
     unsigned pp = 0;  // out(r-2, C-1)
     for (int r = 1; r < R; r++) {
         const long base = (long)r * C;
-        unsigned ppn = prev[C - 1];
-        for (int c = tid; c < C - 1; c += 1024) {
+        uint64_t pr[kMedMaxK];
+#pragma unroll
+        for (int k = 0; k < kMedMaxK; k++) pr[k] = nx[k];
+        const uint64_t prl = nlast;
+        fetch(r + 1);
+        const unsigned ppn = prev[C - 1];
+        unsigned v0 = 0;
+#pragma unroll
+        for (int k = 0; k < kMedMaxK; k++) {
+            const int c = tid + 1024 * k;
+            if (c >= C - 1) break;
             const long q = base + c;
             unsigned v;
             if (q == C) v = 0;
-            else if (q > qhi) v = in[q];
+            else if (q > qhi) v = G(in)[q];
             else {
-                unsigned a = (c == 0) ? pp : prev[c - 1];
-                v = med9_pre(pre[q], a, prev[c], prev[c + 1]);
+                const unsigned a = (c == 0) ? pp : prev[c - 1];
+                v = med9_pre(pr[k], a, prev[c], prev[c + 1]);
             }
             cur[c] = (uint16_t)v;
+            if (k == 0) v0 = v;
         }
-        __syncthreads();
         if (tid == 0) {
+            // column C - 1 reads column 0 of this row (v0)
             const long q = base + C - 1;
-            unsigned v;
-            if (q > qhi) v = in[q];
-            else {
-                v = med9_pre(pre[q], prev[C - 2], prev[C - 1], cur[0]);
-            }
+            const unsigned v = q > qhi ? (unsigned)G(in)[q] : med9_pre(prl, prev[C - 2], prev[C - 1], v0);
             cur[C - 1] = (uint16_t)v;
         }
         __syncthreads();
-        for (int c = tid; c < C; c += 1024) out[base + c] = cur[c];
+        for (int c = tid; c < C; c += 1024) G(out)[base + c] = cur[c];
         pp = ppn;
         uint16_t* t = prev; prev = cur; cur = t;
     }
@@ -706,6 +732,19 @@ __device__ __forceinline__ uint4 add_u16x8(uint4 a, uint4 b)
     return make_uint4(W2(U2(a.x) + U2(b.x)), W2(U2(a.y) + U2(b.y)), W2(U2(a.z) + U2(b.z)), W2(U2(a.w) + U2(b.w)));
 }
 
+// prefetch depth of the family kernels (cells whose loads are in flight per
+// chain); overridable at build time for tuning
+#ifndef SISTER_PF16
+#define SISTER_PF16 4
+#endif
+#ifndef SISTER_PF8
+#define SISTER_PF8 2
+#endif
+template <int CH, int KL, int FAM>
+struct FamTune {
+    static constexpr int PF = KL == 16 ? SISTER_PF16 : (KL == 8 ? SISTER_PF8 : 2);
+};
+
 // Family kernel: FAM 0 rows (lanes over rows, steps along x), 1 columns,
 // 2 diagonals (k = x - y), 3 anti-diagonals (s = x + y, last, with the final
 // winner); each chain swept forward (pass 0 direction) then backward.
@@ -772,7 +811,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
         lane_guard<CH>(st, false);
         const bool need_p = dir == 1;
         const bool need_o = final_pass && dir == 1;
-        auto fetch = [&](int t, RowBuf<CH, NO>& b) {
+        auto fetch = [&](int t, RowBuf<CH, NO>& b) __attribute__((always_inline)) {
             int y, x;
             cell_of(dir, t, y, x);
             const bool act = t < nsteps && is_active(y, x);
@@ -791,7 +830,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                 }
             });
         };
-        auto process = [&](int t, RowBuf<CH, NO>& b) {
+        auto process = [&](int t, RowBuf<CH, NO>& b) __attribute__((always_inline)) {
             int y, x;
             cell_of(dir, t, y, x);
             const bool act = is_active(y, x);
@@ -971,16 +1010,19 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                 if (want_sub) __syncthreads();
             }
         };
-        // software pipeline: the loads of the next cell are in flight while
-        // the current one is computed (deeper prefetch measured no faster)
-        RowBuf<CH, NO> A, B;
-        fetch(0, A);
-        for (int t = 0; t < nsteps; t += 2) {
-            fetch(t + 1, B);
-            process(t, A);
-            if (t + 1 >= nsteps) break;
-            fetch(t + 2, A);
-            process(t + 1, B);
+        // software pipeline: a ring of PF cell buffers, the loads of the next
+        // PF - 1 cells in flight while the current one is computed
+        constexpr int PF = FamTune<CH, KL, FAM>::PF;
+        RowBuf<CH, NO> ring[PF];
+        static_for<0, PF - 1>([&](auto ic) __attribute__((always_inline)) {
+            fetch(decltype(ic)::value, ring[decltype(ic)::value]);
+        });
+        for (int t = 0; t < nsteps; t += PF) {
+            static_for<0, PF>([&](auto ic) __attribute__((always_inline)) {
+                constexpr int i = decltype(ic)::value;
+                fetch(t + i + PF - 1, ring[(i + PF - 1) % PF]);
+                if (t + i < nsteps) process(t + i, ring[i]);
+            });
         }
     }
 }
@@ -1056,6 +1098,8 @@ const char* const kKernelNames[kNumKernels] = {
 
 #define TB(id) do { if (t) t->begin(id, st); } while (0)
 #define TE(id) do { if (t) t->end(id, st); } while (0)
+#define TB_S(id, s) do { if (t) t->begin(id, s); } while (0)
+#define TE_S(id, s) do { if (t) t->end(id, s); } while (0)
 
 void launch_census(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
 {
@@ -1130,23 +1174,42 @@ void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t)
     TE(kKMsum);
 }
 
-void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, KernelTimer* t)
+void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, const FamStreams* fs, KernelTimer* t)
 {
+    // rows on st, columns and diagonals on the side streams, joined before
+    // the anti-diagonal family
+    hipStream_t sc = st, sd = st;
+    if (fs) {
+        (void)hipEventRecord(fs->fork, st);
+        (void)hipStreamWaitEvent(fs->side[0], fs->fork, 0);
+        (void)hipStreamWaitEvent(fs->side[1], fs->fork, 0);
+        sc = fs->side[0];
+        sd = fs->side[1];
+    }
+    for (int gi = 0; gi < ls.ngroups; gi++) {
+        const DGroup& G = ls.groups[gi];
+        TB_S(kKRows, st);
+        SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
+        TE_S(kKRows, st);
+        TB_S(kKCols, sc);
+        SISTER_FAM(G, 1, dim3(G.blocks[1]), dim3(64), 0, sc, G.rigs, G.nrig, G.bs[1], mode, keep_sum);
+        TE_S(kKCols, sc);
+        TB_S(kKDiag, sd);
+        SISTER_FAM(G, 2, dim3(G.blocks[2]), dim3(64), 0, sd, G.rigs, G.nrig, G.bs[2], mode, keep_sum);
+        TE_S(kKDiag, sd);
+    }
+    if (fs) {
+        (void)hipEventRecord(fs->join[0], sc);
+        (void)hipEventRecord(fs->join[1], sd);
+        (void)hipStreamWaitEvent(st, fs->join[0], 0);
+        (void)hipStreamWaitEvent(st, fs->join[1], 0);
+    }
     for (int gi = 0; gi < ls.ngroups; gi++) {
         const DGroup& G = ls.groups[gi];
         const size_t lds = (size_t)64 * G.CH[3] * 32;
-        TB(kKRows);
-        SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
-        TE(kKRows);
-        TB(kKCols);
-        SISTER_FAM(G, 1, dim3(G.blocks[1]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[1], mode, keep_sum);
-        TE(kKCols);
-        TB(kKDiag);
-        SISTER_FAM(G, 2, dim3(G.blocks[2]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[2], mode, keep_sum);
-        TE(kKDiag);
-        TB(kKAnti);
+        TB_S(kKAnti, st);
         SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[3], mode, keep_sum);
-        TE(kKAnti);
+        TE_S(kKAnti, st);
     }
 }
 

@@ -15,6 +15,7 @@
 namespace sister {
 
 constexpr int kMaxDisp = 512;     // packed argmin keys carry d in 9 bits
+constexpr int kMaxSide = 16384;   // padded frame side (the serial median holds a row per thread set)
 constexpr int kP1 = 7;            // sgm(.., 7, 17, 8) at hpp:280
 constexpr int kP2 = 100;          // adaptP2 with alpha = 0, gamma = 100 (sgm.cpp:15-22,34-35)
 constexpr int kLrThreshold = 5;   // doLRCheck(.., 5) at hpp:200

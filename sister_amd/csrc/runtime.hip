@@ -71,7 +71,8 @@ size_t io_bytes(size_t WH)
 
 bool valid_dims(int W, int H, int D)
 {
-    return W >= 4 && H >= 4 && W % 4 == 0 && H % 4 == 0 && D >= 8 && D % 8 == 0 && D <= kMaxDisp;
+    return W >= 4 && H >= 4 && W % 4 == 0 && H % 4 == 0 && D >= 8 && D % 8 == 0 && D <= kMaxDisp &&
+           W + 2 * D <= kMaxSide && H + 2 * D <= kMaxSide;
 }
 
 struct Block {
@@ -205,6 +206,7 @@ struct sister_ctx {
     Plan graph_plan[kMaxParts];
     hipStream_t side[kMaxParts] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kMaxParts] = {};
+    FamStreams fam[kMaxParts] = {};   // per part: streams for concurrent SGM families
     std::mutex mu;             // guards pool / plans between the serving thread and callers
 
     // serving
@@ -260,7 +262,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
     for (int i = 0; i < n; i++) {
         const sister_rig& r = rigs[i];
         if (!valid_dims(r.width, r.height, r.disp_count))
-            return fail(c, SISTER_EINVAL, "rig " + std::to_string(i) + ": need W,H multiples of 4 and D a multiple of 8 in [8,512]");
+            return fail(c, SISTER_EINVAL, "rig " + std::to_string(i) + ": need W,H multiples of 4, D a multiple of 8 in [8,512], W+2D and H+2D at most 16384");
         for (int v = 0; v < 5; v++)
             if (!r.view[v]) return fail(c, SISTER_EINVAL, "rig " + std::to_string(i) + ": null view");
         if (r.stride < r.width) return fail(c, SISTER_EINVAL, "rig " + std::to_string(i) + ": stride < width");
@@ -415,7 +417,7 @@ static void free_plan(sister_ctx* c, Plan& p)
 }
 
 // enqueue every stage of a plan on `st`
-static void enqueue(sister_ctx* c, const Plan& p, int modes, hipStream_t st, KernelTimer* t)
+static void enqueue(sister_ctx* c, const Plan& p, int modes, hipStream_t st, const FamStreams* fs, KernelTimer* t)
 {
     const LaunchSet& ls = p.ls;
     launch_census(ls, st, t);
@@ -430,7 +432,7 @@ static void enqueue(sister_ctx* c, const Plan& p, int modes, hipStream_t st, Ker
     for (int m = 0; m < 3; m++) {
         if (!(modes & (1 << m))) continue;
         launch_msum(ls, fuse[m], st, t);
-        launch_sgm(ls, m, c->keep_sum ? 1 : 0, st, t);
+        launch_sgm(ls, m, c->keep_sum ? 1 : 0, st, fs, t);
     }
 }
 
@@ -466,6 +468,13 @@ int sister_create(const sister_config* cfg, sister_ctx** out)
     for (int k = 1; ok && k < sister_ctx::kMaxParts; k++)
         ok = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < sister_ctx::kMaxParts; k++) {
+        FamStreams& f = c->fam[k];
+        ok = hipEventCreateWithFlags(&f.fork, hipEventDisableTiming) == hipSuccess;
+        for (int j = 0; ok && j < 2; j++)
+            ok = hipStreamCreateWithFlags(&f.side[j], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&f.join[j], hipEventDisableTiming) == hipSuccess;
+    }
     if (!ok) {
         delete c;
         return SISTER_EDEVICE;
@@ -497,6 +506,13 @@ void sister_destroy(sister_ctx* c)
         (void)hipEventDestroy(c->ev_join[k]);
         (void)hipStreamDestroy(c->side[k]);
     }
+    for (auto& f : c->fam) {
+        (void)hipEventDestroy(f.fork);
+        for (int j = 0; j < 2; j++) {
+            (void)hipEventDestroy(f.join[j]);
+            (void)hipStreamDestroy(f.side[j]);
+        }
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -524,7 +540,7 @@ int sister_run_device(sister_ctx* c, const sister_rig* rigs, int n, int modes)
     free_plan(c, c->last);
     rc = build_plan(c, rigs, n, c->last);
     if (rc) return rc;
-    enqueue(c, c->last, modes, c->stream, c->profiling ? &c->timer : nullptr);
+    enqueue(c, c->last, modes, c->stream, &c->fam[0], c->profiling ? &c->timer : nullptr);
     HIPCHK(c, hipGetLastError());
     return SISTER_OK;
 }
@@ -543,7 +559,7 @@ int sister_compute_rig(sister_ctx* c, const sister_rig* r, int modes)
     int rc = check_modes(c, r, 1, modes);
     if (rc) return rc;
     if (!valid_dims(r->width, r->height, r->disp_count))
-        return fail(c, SISTER_EINVAL, "need W,H multiples of 4 and D a multiple of 8 in [8,512]");
+        return fail(c, SISTER_EINVAL, "need W,H multiples of 4, D a multiple of 8 in [8,512], W+2D and H+2D at most 16384");
     if (r->stride < r->width) return fail(c, SISTER_EINVAL, "stride < width");
     const size_t WH = (size_t)r->width * r->height;
     Block io;
@@ -636,7 +652,10 @@ int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     (void)hipEventRecord(c->ev_fork, c->stream);
     for (int h = 1; h < nparts; h++) (void)hipStreamWaitEvent(c->side[h], c->ev_fork, 0);
-    for (int h = 0; h < nparts; h++) enqueue(c, c->graph_plan[h], modes, h ? c->side[h] : c->stream, nullptr);
+    // the SGM families of a part run in sequence inside a captured graph
+    // (concurrent families are used by eager runs only: a capture that forked
+    // them from the part streams ended in a host-side crash in the runtime)
+    for (int h = 0; h < nparts; h++) enqueue(c, c->graph_plan[h], modes, h ? c->side[h] : c->stream, nullptr, nullptr);
     for (int h = 1; h < nparts; h++) {
         (void)hipEventRecord(c->ev_join[h], c->side[h]);
         (void)hipStreamWaitEvent(c->stream, c->ev_join[h], 0);
@@ -831,7 +850,7 @@ static void serve_loop(sister_ctx* c)
                 dev[i] = d;
             }
             if (ok) ok = build_plan(c, dev.data(), (int)dev.size(), f->plan) == SISTER_OK;
-            if (ok) enqueue(c, f->plan, modes, f->stream, nullptr);
+            if (ok) enqueue(c, f->plan, modes, f->stream, nullptr, nullptr);
         }
         for (size_t i = 0; ok && i < batch.size(); i++) {
             const sister_rig& h = batch[i].rig;

@@ -160,7 +160,7 @@ def test_serving_admission_by_budget():
         ctx.close()
 
 
-@pytest.mark.parametrize("W,H,D", [(64, 48, 60), (66, 48, 16), (64, 46, 16), (64, 48, 0), (64, 48, 520)])
+@pytest.mark.parametrize("W,H,D", [(64, 48, 60), (66, 48, 16), (64, 46, 16), (64, 48, 0), (64, 48, 520), (16376, 48, 8)])
 def test_invalid_rigs_rejected(gpu_ctx, W, H, D):
     views = [np.zeros((H, W), np.uint8)] * 5
     with pytest.raises(sister_amd.SisterError) as e:
