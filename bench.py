@@ -108,8 +108,9 @@ def cpu_baseline(rig):
 
 
 def run_single(args):
-    """One eager run of the batch: exactly one dispatch per kernel (for
-    hardware-counter runs under rocprofv3 --pmc)."""
+    """One instrumented eager run of the batch, the same launches whose event
+    times give kernel_ms (one dispatch per kernel and D group, families in
+    sequence): the command profiled by rocprofv3 (--kernel-trace, --pmc)."""
     import torch
     import sister_amd
 
@@ -117,10 +118,14 @@ def run_single(args):
     rigs = make_rigs(args.workload, args.batch)
     dev = to_device(ctx, rigs, torch)
     torch.cuda.synchronize()
+    ctx.profile(True)
     ctx.run_device([d["struct"] for d in dev], 1)
     ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
     cells = sum(padded_cells(r["W"], r["H"], r["D"]) for r in rigs)
-    print(json.dumps({"single_run_rigs": len(rigs), "padded_cells": cells}))
+    print(json.dumps({"single_run_rigs": len(rigs), "padded_cells": cells,
+                      "kernel_ms": {k: ms for k, (ms, n) in prof.items() if n}}))
     ctx.close()
 
 
@@ -294,10 +299,15 @@ def calibrate_copy(gib=1):
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(3):
         torch.add(a, 1.0, out=b)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"calibrate_copy_bytes_read": n * 4, "calibrate_copy_bytes_written": n * 4, "launches": 3}))
+    ms = e0.elapsed_time(e1) / 3
+    print(json.dumps({"calibrate_copy_bytes_read": n * 4, "calibrate_copy_bytes_written": n * 4, "launches": 3,
+                      "ms_per_launch": ms, "copy_GBps": 2 * n * 4 / (ms * 1e-3) / 1e9}))
 
 
 def main():

@@ -540,7 +540,8 @@ int sister_run_device(sister_ctx* c, const sister_rig* rigs, int n, int modes)
     free_plan(c, c->last);
     rc = build_plan(c, rigs, n, c->last);
     if (rc) return rc;
-    enqueue(c, c->last, modes, c->stream, &c->fam[0], c->profiling ? &c->timer : nullptr);
+    // profiling times every kernel on its own: the SGM families then run in sequence
+    enqueue(c, c->last, modes, c->stream, c->profiling ? nullptr : &c->fam[0], c->profiling ? &c->timer : nullptr);
     HIPCHK(c, hipGetLastError());
     return SISTER_OK;
 }
