@@ -248,46 +248,79 @@ def run(args):
 
 
 def run_serve(args):
-    """Poisson arrivals of mixed rigs into the continuous-batching server."""
+    """Continuous batching under Poisson arrivals: the mixed-size pool (640x480
+    to 2048x1536, D 64..256) plus 5% 4096x3000 rigs, HBM budget --hbm-gb.
+    Default: device-resident buffers (sister_submit_device); --serve-host
+    submits host buffers (staged in and out by the server)."""
     import numpy as np
+    import torch
     import sister_amd
     from sister_amd import synth
 
     ctx = sister_amd.Context(device=0, hbm_budget=args.hbm_gb << 30)
     shapes = synth.batch_shapes(64, 1000)
     pool = [synth.rig_scene(W, H, D, 1000 + i) for i, (W, H, D) in enumerate(shapes[:8])]
+    large = synth.rig_scene(4096, 3000, shapes[0][2], 5, n_boxes=20, blur_sigma=0.0)
     rng = np.random.default_rng(7)
+    n = int(args.rate * args.serve_seconds)
+    # request i uses the large rig with probability 0.05, else a pool rig
+    picks = [-1 if rng.random() < 0.05 else int(rng.integers(0, len(pool))) for _ in range(n)]
+    gaps = rng.exponential(1.0 / args.rate, size=n)
+    rig_of = lambda k: large if k < 0 else pool[k]
+    device = not args.serve_host
+    slots = {}   # (pick, slot) -> (struct, keep-alive); two output sets per rig
+    for k in sorted(set(picks)):
+        r = rig_of(k)
+        for sl in range(2):
+            if device:
+                d = to_device(ctx, [r], torch)[0]
+                slots[(k, sl)] = (d["struct"], d)
+            else:
+                o = np.zeros((r["H"], r["W"]), np.uint16)
+                st = sister_amd.RigStruct()
+                st.width, st.height, st.stride, st.disp_count = r["W"], r["H"], r["W"], r["D"]
+                for v in range(5):
+                    st.view[v] = r["views"][v].ctypes.data
+                st.out_multiview = o.ctypes.data
+                slots[(k, sl)] = (st, o)
+    torch.cuda.synchronize()
     ctx.serve_start(4)
-    outs, tickets = [], []
+    tickets = []
     t0 = time.perf_counter()
     tnext = t0
-    n = args.serve_requests
-    for i in range(n):
-        r = pool[i % len(pool)]
-        o = np.zeros((r["H"], r["W"]), np.uint16)
-        s = sister_amd.RigStruct()
-        s.width, s.height, s.stride, s.disp_count = r["W"], r["H"], r["W"], r["D"]
-        for k in range(5):
-            s.view[k] = r["views"][k].ctypes.data
-        s.out_multiview = o.ctypes.data
+    tlog = t0
+    for i, k in enumerate(picks):
         while time.perf_counter() < tnext:
             time.sleep(0.0002)
-        tickets.append(ctx.submit(s, 1))
-        outs.append(o)
-        tnext += rng.exponential(1.0 / args.rate)
-    for t in tickets:
-        ctx.wait(t)
+        tickets.append(ctx.submit(slots[(k, i & 1)][0], 1, device=device))
+        tnext += gaps[i]
+        if time.perf_counter() - tlog > 2.0:
+            tlog = time.perf_counter()
+            log(f"serve: {i + 1}/{n} submitted, {ctx.stats()['completed']} completed")
+    failed, reason = 0, None
+    for i, t in enumerate(tickets):
+        try:
+            ctx.wait(t)
+        except sister_amd.SisterError as e:
+            failed += 1
+            reason = str(e)
+        if time.perf_counter() - tlog > 2.0:
+            tlog = time.perf_counter()
+            log(f"serve: {i + 1}/{n} completed")
     wall = time.perf_counter() - t0
     st = ctx.stats()
     ctx.serve_stop()
     ctx.close()
-    evals = sum(pool[i % len(pool)]["W"] * pool[i % len(pool)]["H"] * pool[i % len(pool)]["D"] for i in range(n))
+    evals = sum(rig_of(k)["W"] * rig_of(k)["H"] * rig_of(k)["D"] for k in picks)
     return {
         "metric": METRIC, "value": evals / wall / 1e9, "unit": "G disparity-evals/s", "n_gpus": 1,
         "steps": n, "warmup": 0, "ms_per_step": wall * 1e3 / n, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
-        "config": {"workload": f"serving: Poisson {args.rate} rigs/s, mixed sizes/D, HBM budget {args.hbm_gb} GiB"},
-        "rigs_per_s": n / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "stats": st,
+        "config": {"workload": f"serving: Poisson {args.rate} rigs/s for {args.serve_seconds} s, mixed sizes/D "
+                               f"+ 5% 4096x3000, {'device-resident' if device else 'host'} buffers, "
+                               f"HBM budget {args.hbm_gb} GiB"},
+        "rigs_per_s": n / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+        "large_rigs": sum(1 for k in picks if k < 0), "failed": failed, "fail_reason": reason, "stats": st,
     }
 
 
@@ -319,7 +352,8 @@ def main():
     ap.add_argument("--workload", default="rig1080", choices=["rig1080", "mixed32", "large", "serve"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rate", type=float, default=200.0)
-    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-host", action="store_true", help="serve host buffers instead of device-resident ones")
     ap.add_argument("--hbm-gb", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--single", action="store_true", help="one eager run of the batch (counter runs)")

@@ -112,6 +112,12 @@ typedef struct {
    completion. */
 int sister_serve_start(sister_ctx* ctx, int max_inflight);
 int sister_submit(sister_ctx* ctx, const sister_rig* rig, int modes, uint64_t* ticket);
+/* The same request with every buffer of the rig in device memory of the
+   context's GPU (the ROS callback of ros/README.md:47-56 with its frames
+   already on the GPU): nothing is staged or copied; the outputs are written to
+   the rig's device buffers, which must stay valid until completion.  The
+   footprint charged against the budget is the workspace alone. */
+int sister_submit_device(sister_ctx* ctx, const sister_rig* rig, int modes, uint64_t* ticket);
 int sister_poll(sister_ctx* ctx, uint64_t ticket, int* done);
 int sister_wait(sister_ctx* ctx, uint64_t ticket);
 int sister_serve_stop(sister_ctx* ctx);

@@ -68,6 +68,7 @@ _SIGS = {
     "sister_graph_launch": (C.c_int, [C.c_void_p]),
     "sister_serve_start": (C.c_int, [C.c_void_p, C.c_int]),
     "sister_submit": (C.c_int, [C.c_void_p, C.POINTER(RigStruct), C.c_int, C.POINTER(C.c_uint64)]),
+    "sister_submit_device": (C.c_int, [C.c_void_p, C.POINTER(RigStruct), C.c_int, C.POINTER(C.c_uint64)]),
     "sister_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)]),
     "sister_wait": (C.c_int, [C.c_void_p, C.c_uint64]),
     "sister_serve_stop": (C.c_int, [C.c_void_p]),
@@ -215,9 +216,12 @@ class Context:
     def serve_start(self, max_inflight=4):
         self._check(lib().sister_serve_start(self._h, max_inflight))
 
-    def submit(self, rig, modes=MODE_MULTIVIEW):
+    def submit(self, rig, modes=MODE_MULTIVIEW, device=False):
+        """Queue a rig; device=True when every buffer of the rig is device
+        memory (computed in place, no staging copies)."""
         t = C.c_uint64()
-        self._check(lib().sister_submit(self._h, C.byref(rig), modes, C.byref(t)))
+        f = lib().sister_submit_device if device else lib().sister_submit
+        self._check(f(self._h, C.byref(rig), modes, C.byref(t)))
         return t.value
 
     def poll(self, ticket):

@@ -25,16 +25,14 @@
 
 #include "../../include/sister.h"
 #include "kernels.h"
+#include "pool.h"
 #include "rig.h"
 
 using namespace sister;
 
 namespace {
 
-constexpr size_t kDefaultBudget = (size_t)64 << 30;
-constexpr size_t kAlign = 256;
 
-inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 // workspace layout of one rig (offsets from the workspace base)
 struct WsLayout {
@@ -75,59 +73,41 @@ bool valid_dims(int W, int H, int D)
            W + 2 * D <= kMaxSide && H + 2 * D <= kMaxSide;
 }
 
-struct Block {
-    char* ptr;
-    size_t bytes;
-};
-
-// Simple caching allocator bounded by the HBM budget.
-struct Pool {
-    size_t budget = kDefaultBudget;
-    size_t in_use = 0, held = 0, peak = 0;
+// Pinned host buffers for the serving path's staging, cached by size (a
+// request's inputs are copied into one with the CPU, then sent with a single
+// asynchronous copy; outputs come back the same way).  Freed at destroy.
+struct PinnedPool {
     std::multimap<size_t, char*> idle;
-
-    hipError_t get(size_t bytes, Block& out)
+    std::vector<char*> all;
+    char* get(size_t bytes)
     {
-        bytes = align_up(bytes);
         auto it = idle.lower_bound(bytes);
-        if (it != idle.end() && it->first <= bytes * 2) {
-            out = {it->second, it->first};
+        if (it != idle.end() && it->first <= 2 * bytes) {
+            char* p = it->second;
             idle.erase(it);
-            in_use += out.bytes;
-            peak = std::max(peak, in_use);
-            return hipSuccess;
+            return p;
         }
-        if (held + bytes > budget) trim();
-        if (held + bytes > budget) return hipErrorOutOfMemory;
         char* p = nullptr;
-        hipError_t e = hipMalloc((void**)&p, bytes);
-        if (e != hipSuccess) {
+        if (hipHostMalloc((void**)&p, bytes, hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
-            trim();
-            e = hipMalloc((void**)&p, bytes);
-            if (e != hipSuccess) { (void)hipGetLastError(); return e; }
+            return nullptr;
         }
-        held += bytes;
-        in_use += bytes;
-        peak = std::max(peak, in_use);
-        out = {p, bytes};
-        return hipSuccess;
+        all.push_back(p);
+        sizes[p] = bytes;
+        return p;
     }
-    void put(const Block& b)
+    void put(char* p)
     {
-        if (!b.ptr) return;
-        in_use -= b.bytes;
-        idle.emplace(b.bytes, b.ptr);
-    }
-    void trim()
-    {
-        for (auto& kv : idle) { (void)hipFree(kv.second); held -= kv.first; }
-        idle.clear();
+        if (p) idle.emplace(sizes[p], p);
     }
     void release_all()
     {
-        trim();
+        for (char* p : all) (void)hipHostFree(p);
+        all.clear();
+        idle.clear();
+        sizes.clear();
     }
+    std::unordered_map<char*, size_t> sizes;
 };
 
 // HIP events around every kernel, accumulated per kernel id.
@@ -214,21 +194,25 @@ struct sister_ctx {
         uint64_t ticket;
         sister_rig rig;
         int modes;
+        bool device;          // rig buffers are device memory (no staging)
         std::chrono::steady_clock::time_point t_submit;
         size_t footprint;
     };
     struct Inflight {
         std::vector<Request> reqs;
         Plan plan;
-        std::vector<Block> io;
+        std::vector<Block> io;         // device staging of host requests
+        std::vector<char*> pinned;     // pinned host staging, per request (null for device requests)
         hipStream_t stream;
         hipEvent_t done;
     };
+    PinnedPool pinned;
     std::thread server;
     std::mutex smu;
     std::condition_variable scv;
     std::deque<Request> queue;
     std::unordered_map<uint64_t, int> status;   // 0 queued, 1 running, 2 done, 3 failed
+    std::string serve_error;                     // reason of the last failed request
     bool serving = false, stop = false;
     int max_inflight = 4;
     uint64_t next_ticket = 1;
@@ -283,7 +267,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         for (int v = 0; v < 5; v++) g.view[v] = r.view[v];
         WsLayout l = ws_layout(g.W, g.H, g.D, c->keep_sum);
         Block b;
-        if (c->pool.get(l.total, b) != hipSuccess) {
+        if (c->pool.get(l.total, b) != 0) {
             for (auto& w : p.workspaces) c->pool.put(w);
             p.workspaces.clear();
             return fail(c, SISTER_ENOMEM, "workspace of " + std::to_string(l.total) + " bytes exceeds the HBM budget");
@@ -372,7 +356,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         for (int f = 0; f < kNumGroupKernels; f++) gbs[gpos + f * (ng + 1) + ng] = tot[f];
         gpos += kNumGroupKernels * (size_t)(ng + 1);
     }
-    if (c->pool.get(host.size(), p.table) != hipSuccess) {
+    if (c->pool.get(host.size(), p.table) != 0) {
         for (auto& w : p.workspaces) c->pool.put(w);
         p.workspaces.clear();
         return fail(c, SISTER_ENOMEM, "plan table allocation failed");
@@ -455,6 +439,12 @@ int sister_create(const sister_config* cfg, sister_ctx** out)
     *out = nullptr;
     sister_ctx* c = new sister_ctx();
     c->device = cfg ? cfg->device : 0;
+    c->pool.dev_alloc = [](char** p, size_t n) {
+        if (hipMalloc((void**)p, n) == hipSuccess) return 0;
+        (void)hipGetLastError();
+        return -1;
+    };
+    c->pool.dev_free = [](char* p) { (void)hipFree(p); };
     if (cfg && cfg->hbm_budget) c->pool.budget = cfg->hbm_budget;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
@@ -501,6 +491,7 @@ void sister_destroy(sister_ctx* c)
     for (auto& gp : c->graph_plan) free_plan(c, gp);
     c->timer.collect();
     c->pool.release_all();
+    c->pinned.release_all();
     (void)hipEventDestroy(c->ev_fork);
     for (int k = 1; k < sister_ctx::kMaxParts; k++) {
         (void)hipEventDestroy(c->ev_join[k]);
@@ -567,7 +558,7 @@ int sister_compute_rig(sister_ctx* c, const sister_rig* r, int modes)
     {
         std::lock_guard<std::mutex> lk(c->mu);
         HIPCHK(c, hipSetDevice(c->device));
-        if (c->pool.get(io_bytes(WH), io) != hipSuccess)
+        if (c->pool.get(io_bytes(WH), io) != 0)
             return fail(c, SISTER_ENOMEM, "staging allocation failed");
     }
     char* p = io.ptr;
@@ -738,10 +729,10 @@ int sister_debug_keep_sum(sister_ctx* c, int on)
 // ---------------------------------------------------------------------------
 // serving
 // ---------------------------------------------------------------------------
-static size_t rig_footprint(const sister_rig& r)
+static size_t rig_footprint(const sister_rig& r, bool device)
 {
     const size_t WH = (size_t)r.width * r.height;
-    return ws_layout(r.width, r.height, r.disp_count).total + io_bytes(WH) + 16 * kAlign;
+    return ws_layout(r.width, r.height, r.disp_count).total + (device ? 0 : io_bytes(WH)) + 16 * kAlign;
 }
 
 static void serve_loop(sister_ctx* c)
@@ -755,22 +746,43 @@ static void serve_loop(sister_ctx* c)
         streams.push_back(s);
     }
     size_t reserved = 0;   // footprint of admitted requests
+    // Admission counts footprints against the budget; the arena may still not
+    // place a batch (chunk rounding, fragmentation).  Such a batch goes back to
+    // the queue, and when nothing else is running the next one is halved.
+    size_t admit_cap = (size_t)-1;
     for (;;) {
         // retire finished batches
         for (size_t i = 0; i < inflight.size();) {
             sister_ctx::Inflight* f = inflight[i];
             hipError_t q = hipEventQuery(f->done);
             if (q == hipErrorNotReady) { i++; continue; }
+            // host requests: outputs from the pinned staging to the caller
+            for (size_t k = 0; k < f->reqs.size(); k++) {
+                char* hp = f->pinned[k];
+                if (!hp || q != hipSuccess) continue;
+                const sister_rig& h = f->reqs[k].rig;
+                const int modes = f->reqs[k].modes;
+                const size_t WH = (size_t)h.width * h.height;
+                const char* o = hp + 5 * align_up(WH);
+                const size_t m2 = align_up(WH * 2);
+                if (h.out_multiview && (modes & 1)) std::memcpy(h.out_multiview, o, WH * 2);
+                if (h.out_horizontal && (modes & 2)) std::memcpy(h.out_horizontal, o + m2, WH * 2);
+                if (h.out_vertical && (modes & 4)) std::memcpy(h.out_vertical, o + 2 * m2, WH * 2);
+                if (h.out_disp && (modes & 1)) std::memcpy(h.out_disp, o + 3 * m2, WH * 2);
+                if (h.out_subpix && (modes & 1)) std::memcpy(h.out_subpix, o + 4 * m2, WH * 4);
+            }
             auto now = std::chrono::steady_clock::now();
             {
                 std::lock_guard<std::mutex> lk(c->mu);
                 free_plan(c, f->plan);
                 for (auto& b : f->io) c->pool.put(b);
             }
+            for (char* hp : f->pinned) c->pinned.put(hp);
             {
                 std::lock_guard<std::mutex> lk(c->smu);
                 for (auto& r : f->reqs) {
                     c->status[r.ticket] = (q == hipSuccess) ? 2 : 3;
+                    if (q != hipSuccess) c->serve_error = hipGetErrorString(q);
                     c->lat_ms.push_back(std::chrono::duration<double, std::milli>(now - r.t_submit).count());
                     reserved -= r.footprint;
                     c->n_done++;
@@ -789,10 +801,11 @@ static void serve_loop(sister_ctx* c)
             std::unique_lock<std::mutex> lk(c->smu);
             if (c->stop && c->queue.empty() && inflight.empty()) break;
             if (!streams.empty()) {
-                while (!c->queue.empty()) {
+                while (!c->queue.empty() && batch.size() < admit_cap) {
                     auto& r = c->queue.front();
                     if (reserved + r.footprint > c->pool.budget) {
                         if (inflight.empty() && batch.empty()) {   // can never fit
+                            c->serve_error = "footprint exceeds the HBM budget";
                             c->status[r.ticket] = 3;
                             c->n_rej++;
                             c->queue.pop_front();
@@ -827,19 +840,39 @@ static void serve_loop(sister_ctx* c)
         bool ok = true;
         int modes = 0;
         for (auto& r : batch) modes |= r.modes;
+        bool no_memory = false;
+        std::string fail_reason;
         {
             std::lock_guard<std::mutex> lk(c->mu);
+            f->pinned.assign(batch.size(), nullptr);
             for (size_t i = 0; i < batch.size() && ok; i++) {
                 const sister_rig& h = batch[i].rig;
+                if (batch[i].device) {   // computed in place
+                    dev[i] = h;
+                    continue;
+                }
                 const size_t WH = (size_t)h.width * h.height;
                 Block b;
-                if (c->pool.get(io_bytes(WH), b) != hipSuccess) { ok = false; break; }
+                if (c->pool.get(io_bytes(WH), b) != 0) { ok = false; no_memory = true; break; }
                 f->io.push_back(b);
+                char* hp = c->pinned.get(io_bytes(WH));
+                if (!hp) { ok = false; break; }
+                f->pinned[i] = hp;
+                // views into the pinned buffer (CPU), then one asynchronous copy
+                for (int v = 0; v < 5; v++) {
+                    char* dst = hp + v * align_up(WH);
+                    if (h.stride == h.width) {
+                        std::memcpy(dst, h.view[v], WH);
+                    } else {
+                        for (int y = 0; y < h.height; y++)
+                            std::memcpy(dst + (size_t)y * h.width, h.view[v] + (size_t)y * h.stride, h.width);
+                    }
+                }
+                (void)hipMemcpyAsync(b.ptr, hp, 5 * align_up(WH), hipMemcpyHostToDevice, f->stream);
                 char* p = b.ptr;
                 sister_rig d = h;
                 d.stride = h.width;
                 for (int v = 0; v < 5; v++) {
-                    (void)hipMemcpy2DAsync(p, h.width, h.view[v], h.stride, h.width, h.height, hipMemcpyHostToDevice, f->stream);
                     d.view[v] = (const uint8_t*)p;
                     p += align_up(WH);
                 }
@@ -850,34 +883,60 @@ static void serve_loop(sister_ctx* c)
                 d.out_subpix = (float*)p;
                 dev[i] = d;
             }
-            if (ok) ok = build_plan(c, dev.data(), (int)dev.size(), f->plan) == SISTER_OK;
+            if (ok) {
+                const int rc = build_plan(c, dev.data(), (int)dev.size(), f->plan);
+                ok = rc == SISTER_OK;
+                no_memory = rc == SISTER_ENOMEM;
+                if (!ok) fail_reason = c->err;
+            } else {
+                fail_reason = "staging allocation failed";
+            }
             if (ok) enqueue(c, f->plan, modes, f->stream, nullptr, nullptr);
         }
+        // outputs of host requests back into their pinned staging (copied to
+        // the caller's buffers when the batch retires)
         for (size_t i = 0; ok && i < batch.size(); i++) {
+            if (batch[i].device) continue;
             const sister_rig& h = batch[i].rig;
             const sister_rig& d = dev[i];
             const size_t WH = (size_t)h.width * h.height;
-            if (h.out_multiview && (modes & 1)) (void)hipMemcpyAsync(h.out_multiview, d.out_multiview, WH * 2, hipMemcpyDeviceToHost, f->stream);
-            if (h.out_horizontal && (modes & 2)) (void)hipMemcpyAsync(h.out_horizontal, d.out_horizontal, WH * 2, hipMemcpyDeviceToHost, f->stream);
-            if (h.out_vertical && (modes & 4)) (void)hipMemcpyAsync(h.out_vertical, d.out_vertical, WH * 2, hipMemcpyDeviceToHost, f->stream);
-            if (h.out_disp && (modes & 1)) (void)hipMemcpyAsync(h.out_disp, d.out_disp, WH * 2, hipMemcpyDeviceToHost, f->stream);
-            if (h.out_subpix && (modes & 1)) (void)hipMemcpyAsync(h.out_subpix, d.out_subpix, WH * 4, hipMemcpyDeviceToHost, f->stream);
+            char* o = f->pinned[i] + 5 * align_up(WH);
+            const size_t m2 = align_up(WH * 2);
+            const int rm = batch[i].modes;
+            if (h.out_multiview && (rm & 1)) (void)hipMemcpyAsync(o, d.out_multiview, WH * 2, hipMemcpyDeviceToHost, f->stream);
+            if (h.out_horizontal && (rm & 2)) (void)hipMemcpyAsync(o + m2, d.out_horizontal, WH * 2, hipMemcpyDeviceToHost, f->stream);
+            if (h.out_vertical && (rm & 4)) (void)hipMemcpyAsync(o + 2 * m2, d.out_vertical, WH * 2, hipMemcpyDeviceToHost, f->stream);
+            if (h.out_disp && (rm & 1)) (void)hipMemcpyAsync(o + 3 * m2, d.out_disp, WH * 2, hipMemcpyDeviceToHost, f->stream);
+            if (h.out_subpix && (rm & 1)) (void)hipMemcpyAsync(o + 4 * m2, d.out_subpix, WH * 4, hipMemcpyDeviceToHost, f->stream);
         }
         (void)hipEventCreateWithFlags(&f->done, hipEventDisableTiming);
         (void)hipEventRecord(f->done, f->stream);
         if (!ok) {
             (void)hipStreamSynchronize(f->stream);
+            for (char* hp : f->pinned) c->pinned.put(hp);
             std::lock_guard<std::mutex> lk(c->mu);
             free_plan(c, f->plan);
             for (auto& b : f->io) c->pool.put(b);
             std::lock_guard<std::mutex> lk2(c->smu);
-            for (auto& r : f->reqs) { c->status[r.ticket] = 3; reserved -= r.footprint; c->n_rej++; }
+            if (no_memory && (!inflight.empty() || f->reqs.size() > 1)) {
+                // wait for running batches to return memory, or retry smaller
+                if (inflight.empty()) admit_cap = std::max<size_t>(1, f->reqs.size() / 2);
+                for (auto it = f->reqs.rbegin(); it != f->reqs.rend(); ++it) {
+                    c->status[it->ticket] = 0;
+                    reserved -= it->footprint;
+                    c->queue.push_front(*it);
+                }
+            } else {
+                c->serve_error = fail_reason;
+                for (auto& r : f->reqs) { c->status[r.ticket] = 3; reserved -= r.footprint; c->n_rej++; }
+            }
             c->scv.notify_all();
             (void)hipEventDestroy(f->done);
             streams.push_back(f->stream);
             delete f;
             continue;
         }
+        admit_cap = (size_t)-1;
         inflight.push_back(f);
     }
     for (auto s : streams) (void)hipStreamDestroy(s);
@@ -900,21 +959,27 @@ int sister_serve_start(sister_ctx* c, int max_inflight)
     return SISTER_OK;
 }
 
-int sister_submit(sister_ctx* c, const sister_rig* r, int modes, uint64_t* ticket)
+static int submit(sister_ctx* c, const sister_rig* r, int modes, uint64_t* ticket, bool device)
 {
     if (!c || !r || !ticket) return fail(c, SISTER_EINVAL, "bad arguments");
     if (!valid_dims(r->width, r->height, r->disp_count)) return fail(c, SISTER_EINVAL, "invalid rig shape or D");
     if (modes <= 0 || modes > 7) return fail(c, SISTER_EINVAL, "bad modes");
+    if (r->stride < r->width) return fail(c, SISTER_EINVAL, "stride < width");
     for (int v = 0; v < 5; v++)
         if (!r->view[v]) return fail(c, SISTER_EINVAL, "null view");
+    if (device) {
+        const int rc = check_modes(c, r, 1, modes);
+        if (rc) return rc;
+    }
     std::lock_guard<std::mutex> lk(c->smu);
     if (!c->serving) return fail(c, SISTER_EINVAL, "serving not started");
     sister_ctx::Request q;
     q.ticket = c->next_ticket++;
     q.rig = *r;
     q.modes = modes;
+    q.device = device;
     q.t_submit = std::chrono::steady_clock::now();
-    q.footprint = rig_footprint(*r);
+    q.footprint = rig_footprint(*r, device);
     if (q.footprint > c->pool.budget) {
         c->n_rej++;
         return fail(c, SISTER_EBUSY, "rig footprint exceeds the HBM budget");
@@ -928,6 +993,16 @@ int sister_submit(sister_ctx* c, const sister_rig* r, int modes, uint64_t* ticke
     return SISTER_OK;
 }
 
+int sister_submit(sister_ctx* c, const sister_rig* r, int modes, uint64_t* ticket)
+{
+    return submit(c, r, modes, ticket, false);
+}
+
+int sister_submit_device(sister_ctx* c, const sister_rig* r, int modes, uint64_t* ticket)
+{
+    return submit(c, r, modes, ticket, true);
+}
+
 int sister_poll(sister_ctx* c, uint64_t t, int* done)
 {
     if (!c || !done) return SISTER_EINVAL;
@@ -935,7 +1010,7 @@ int sister_poll(sister_ctx* c, uint64_t t, int* done)
     auto it = c->status.find(t);
     if (it == c->status.end()) return fail(c, SISTER_ENOTFOUND, "unknown ticket");
     *done = it->second >= 2;
-    return it->second == 3 ? fail(c, SISTER_EDEVICE, "request failed") : SISTER_OK;
+    return it->second == 3 ? fail(c, SISTER_EDEVICE, "request failed: " + c->serve_error) : SISTER_OK;
 }
 
 int sister_wait(sister_ctx* c, uint64_t t)
@@ -945,7 +1020,7 @@ int sister_wait(sister_ctx* c, uint64_t t)
     auto it = c->status.find(t);
     if (it == c->status.end()) return fail(c, SISTER_ENOTFOUND, "unknown ticket");
     c->scv.wait(lk, [&] { return c->status[t] >= 2; });
-    return c->status[t] == 3 ? fail(c, SISTER_EDEVICE, "request failed") : SISTER_OK;
+    return c->status[t] == 3 ? fail(c, SISTER_EDEVICE, "request failed: " + c->serve_error) : SISTER_OK;
 }
 
 int sister_serve_stop(sister_ctx* c)

@@ -130,6 +130,33 @@ def test_serving_alone_equals_mix():
         ctx.close()
 
 
+def test_serving_device_buffers_equal_run_device(gpu_ctx):
+    """Device-resident submissions: computed in place, equal to the same rigs
+    run directly."""
+    import torch
+    rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in MIX]
+    ref = [_device_rig(gpu_ctx, r, torch) for r in rigs]
+    gpu_ctx.run_device([d["struct"] for d in ref], 1)
+    gpu_ctx.sync()
+    ctx = sister_amd.Context()
+    try:
+        ctx.serve_start(3)
+        dev = [_device_rig(ctx, r, torch) for r in rigs for _ in range(2)]
+        torch.cuda.synchronize()
+        print("[serve-device] 12 device-resident requests", flush=True)
+        tickets = [ctx.submit(d["struct"], 1, device=True) for d in dev]
+        for t in tickets:
+            ctx.wait(t)
+        for i, d in enumerate(dev):
+            assert torch.equal(d["outs"]["disp"], ref[i // 2]["outs"]["disp"]), i
+            assert torch.equal(d["outs"]["multiview"], ref[i // 2]["outs"]["multiview"]), i
+        st = ctx.stats()
+        assert st["completed"] == 12 and st["rejected"] == 0
+        ctx.serve_stop()
+    finally:
+        ctx.close()
+
+
 def test_serving_admission_by_budget():
     small = synth.rig_small(64, 48, 16, 71)
     big = synth.rig_small(320, 240, 64, 72)
