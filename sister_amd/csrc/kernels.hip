@@ -301,11 +301,134 @@ __device__ __forceinline__ unsigned med9_pre(uint64_t s, unsigned a, unsigned b,
 }
 
 // One block per (rig, map), 1024 threads; rows in sequence, columns in
-// parallel (thread t owns columns t, t + 1024, ...; frame rows are at most
-// kMedMaxK * 1024 wide).  The order statistics of row r + 1 are loaded while
-// row r is computed.  The last column of a row reads column 0 of the same row,
-// which thread 0 has just computed itself, so one barrier per row suffices.
+// parallel: thread t owns columns t + 1024 k for k < K = ceil(C / 1024), and
+// frame rows are at most kMedMaxK * 1024 wide.  The order statistics of row
+// r + 1 are loaded while row r is computed (two register buffers,
+// alternating), and each result is stored from the register that holds it.
+// LDS keeps two rows of results, each with one slot in front (index -1)
+// holding out(r - 2, C - 1), the raster-order predecessor of column 0 in the
+// window of row r; so every column reads its three upper neighbours alike.
+// The last column reads column 0 of the same row, which thread 0 computes
+// itself, so one barrier per row suffices.  Rows 2 .. R - 3 have none of the
+// special cases (out(1, 0) = 0 and the copied tail from index R*C - C - 4 on)
+// and run without checks.  EXACT: K is the frame's exact slot count, so only
+// slot K - 1 can reach past column C - 2.  PF: load row r + 1 during row r
+// (frames up to 8192 wide); without it the loads of a row precede its
+// compute, which keeps the 16-slot buffer of the widest frames in registers.
 constexpr int kMedMaxK = 16;
+
+template <int K, bool EXACT, bool PF>
+__device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t* __restrict__ in,
+                                                   uint16_t* __restrict__ out,
+                                                   const uint64_t* __restrict__ pre, uint16_t* lds)
+{
+    const int tid = threadIdx.x;
+    const bool wave0 = __builtin_amdgcn_readfirstlane(tid) < 64;
+    const int qhi = R * C - C - 5;
+    const int kact = (C + 1023) >> 10;
+    uint16_t* prev = lds + 1;
+    uint16_t* cur = lds + C + 2;
+    for (int c = tid; c < C; c += 1024) {
+        const uint16_t v = G(in)[c];
+        prev[c] = v;
+        G(out)[c] = v;
+    }
+    if (tid == 0) prev[-1] = 0;
+    // slot k may hold a column below C - 1 (one that this thread computes)
+    auto live = [&](int k) __attribute__((always_inline)) {
+        return (EXACT && k < K - 1) || tid + 1024 * k < C - 1;
+    };
+    auto col = [&](int k) __attribute__((always_inline)) {
+        return (EXACT && k < K - 1) ? tid + 1024 * k : min(tid + 1024 * k, C - 2);
+    };
+    // Order statistics of row r (1 <= r < R).  Indices past qhi (the copied
+    // tail) are clamped; their statistics are loaded but never used.  Byte
+    // offsets fit 32 bits (R * C * 8 <= 2^31).  The last column's statistics
+    // are needed by thread 0 alone; wave 0 loads them with a lane-dependent
+    // index (lane 0's is the one used) so that the load stays a vector load:
+    // a scalar load would make every LDS wait of the row wait for it too.
+    auto at = [&](int i) __attribute__((always_inline)) {
+        return G((const uint64_t*)((const char*)pre + (uint32_t)i * 8u));
+    };
+    auto fetch = [&](int r, uint64_t (&nx)[K], uint64_t& nlast) __attribute__((always_inline)) {
+        const int base = r * C;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!EXACT && k >= kact) break;
+            nx[k] = *at(min(base + col(k), qhi));
+        }
+        if (wave0) nlast = *at(min(base + C - 1 + tid, qhi));
+    };
+    // In rows 2 .. R - 3 nothing is read from `in`; the other rows wait for
+    // their `in` loads inside their own branch (vmcnt(0), expcnt and lgkmcnt
+    // left free: 0x0F70), so that the plain rows' stores do not wait for the
+    // next row's statistics still in flight.
+    auto row = [&](int r, const uint64_t (&pr)[K], uint64_t prl) __attribute__((always_inline)) {
+        const int base = r * C;
+        const bool plain = r >= 2 && r <= R - 3;
+        unsigned v[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!EXACT && k >= kact) break;
+            const int c = col(k);
+            v[k] = med9_pre(pr[k], prev[c - 1], prev[c], prev[c + 1]);
+        }
+        if (!plain) {
+            // out(1, 0) = 0; from index qhi + 1 on the input is copied
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                if (!EXACT && k >= kact) break;
+                const int q = base + tid + 1024 * k;
+                if (q == C) v[k] = 0;
+                else if (q > qhi && live(k)) v[k] = G(in)[q];
+            }
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!EXACT && k >= kact) break;
+            if (live(k)) {
+                const int c = tid + 1024 * k;
+                cur[c] = (uint16_t)v[k];
+                G(out)[base + c] = (uint16_t)v[k];
+            }
+        }
+        if (tid == 0) {
+            // column C - 1 reads column 0 of this row, v[0] of this thread
+            const int q = base + C - 1;
+            unsigned w = med9_pre(prl, prev[C - 2], prev[C - 1], v[0]);
+            if (!plain) {
+                if (q > qhi) w = G(in)[q];
+                __builtin_amdgcn_s_waitcnt(0x0F70);
+            }
+            cur[C - 1] = (uint16_t)w;
+            cur[-1] = prev[C - 1];
+            G(out)[q] = (uint16_t)w;
+        }
+        __syncthreads();
+        uint16_t* t = prev; prev = cur; cur = t;
+    };
+    if constexpr (PF) {
+        uint64_t bufA[K], bufB[K], lastA, lastB;
+        fetch(1, bufA, lastA);
+        __syncthreads();
+        int r = 1;
+        for (; r + 1 < R; r += 2) {
+            fetch(r + 1, bufB, lastB);
+            row(r, bufA, lastA);
+            fetch(min(r + 2, R - 1), bufA, lastA);
+            row(r + 1, bufB, lastB);
+        }
+        if (r < R) row(r, bufA, lastA);
+    } else {
+        uint64_t buf[K], last;
+        __syncthreads();
+        for (int r = 1; r < R; r++) {
+            fetch(r, buf, last);
+            row(r, buf, last);
+        }
+    }
+}
 
 __global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restrict__ rigs,
                                                         uint64_t* const* __restrict__ scratch)
@@ -315,69 +438,16 @@ __global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restric
     const RigDesc& g = rigs[ri];
     const int F = m >> 1;
     const int R = frame_rows(g, F), C = frame_cols(g, F);
-    const long npx = (long)R * C;
     const uint16_t* in = g.wta[m];
     uint16_t* out = g.med[m];
-    const uint64_t* pre = scratch[ri] + (size_t)m * npx;
-    const long qhi = npx - C - 5;
-    uint16_t* prev = lds16;
-    uint16_t* cur = lds16 + C;
-    const int tid = threadIdx.x;
-    for (int c = tid; c < C; c += 1024) {
-        uint16_t v = G(in)[c];
-        prev[c] = v;
-        G(out)[c] = v;
-    }
-    // order statistics of a row: columns tid + 1024 k (k < kMedMaxK) below
-    // C - 1, and for thread 0 the last column too
-    uint64_t nx[kMedMaxK], nlast = 0;
-    auto fetch = [&](int r) __attribute__((always_inline)) {
-        const long base = (long)r * C;
-#pragma unroll
-        for (int k = 0; k < kMedMaxK; k++) {
-            const int c = tid + 1024 * k;
-            const long q = base + c;
-            nx[k] = (r < R && c < C - 1 && q <= qhi) ? G(pre)[q] : 0;
-        }
-        if (tid == 0) nlast = (r < R && base + C - 1 <= qhi) ? G(pre)[base + C - 1] : 0;
-    };
-    fetch(1);
-    __syncthreads();
-    unsigned pp = 0;  // out(r-2, C-1)
-    for (int r = 1; r < R; r++) {
-        const long base = (long)r * C;
-        uint64_t pr[kMedMaxK];
-#pragma unroll
-        for (int k = 0; k < kMedMaxK; k++) pr[k] = nx[k];
-        const uint64_t prl = nlast;
-        fetch(r + 1);
-        const unsigned ppn = prev[C - 1];
-        unsigned v0 = 0;
-#pragma unroll
-        for (int k = 0; k < kMedMaxK; k++) {
-            const int c = tid + 1024 * k;
-            if (c >= C - 1) break;
-            const long q = base + c;
-            unsigned v;
-            if (q == C) v = 0;
-            else if (q > qhi) v = G(in)[q];
-            else {
-                const unsigned a = (c == 0) ? pp : prev[c - 1];
-                v = med9_pre(pr[k], a, prev[c], prev[c + 1]);
-            }
-            cur[c] = (uint16_t)v;
-            if (k == 0) v0 = v;
-        }
-        if (tid == 0) {
-            // column C - 1 reads column 0 of this row (v0)
-            const long q = base + C - 1;
-            const unsigned v = q > qhi ? (unsigned)G(in)[q] : med9_pre(prl, prev[C - 2], prev[C - 1], v0);
-            cur[C - 1] = (uint16_t)v;
-        }
-        __syncthreads();
-        for (int c = tid; c < C; c += 1024) G(out)[base + c] = cur[c];
-        pp = ppn;
-        uint16_t* t = prev; prev = cur; cur = t;
+    const uint64_t* pre = scratch[ri] + (size_t)m * R * C;
+    switch ((C + 1023) >> 10) {
+    case 1: median_serial_body<1, true, true>(R, C, in, out, pre, lds16); break;
+    case 2: median_serial_body<2, true, true>(R, C, in, out, pre, lds16); break;
+    case 3: median_serial_body<3, true, true>(R, C, in, out, pre, lds16); break;
+    case 4: median_serial_body<4, true, true>(R, C, in, out, pre, lds16); break;
+    case 5: case 6: case 7: case 8: median_serial_body<8, false, true>(R, C, in, out, pre, lds16); break;
+    default: median_serial_body<kMedMaxK, false, false>(R, C, in, out, pre, lds16); break;
     }
 }
 
@@ -1122,7 +1192,7 @@ void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
     hipLaunchKernelGGL(k_median_prep, dim3(ls.blocks[kStageMedianPrep]), dim3(256), 0, st, ls.rigs, ls.nrig,
                        ls.bs[kStageMedianPrep], ls.scratch);
     TE(kKMedianPrep);
-    size_t lds = (size_t)2 * ls.maxCols * sizeof(uint16_t);
+    size_t lds = (size_t)2 * (ls.maxCols + 1) * sizeof(uint16_t);
     TB(kKMedianSerial);
     hipLaunchKernelGGL(k_median_serial, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
     TE(kKMedianSerial);
