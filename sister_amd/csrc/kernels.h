@@ -18,6 +18,7 @@ enum Stage {
 enum GroupKernel {
     kGRows = 0, kGCols, kGDiag, kGAnti,   // SGM families
     kGMsumRows, kGMsumCols,               // cost sum: frames 0/180 (row runs), 90/270 (column runs)
+    kGMsumTile,                           // cost sum: all frames, per 16x16 tile
     kNumGroupKernels
 };
 struct DGroup {
@@ -61,7 +62,10 @@ void stage_blocks(const RigDesc& g, int maxD, int out[kNumStages]);
 int sgm_pick_kl(long cols_chains, int fam, int D);
 int sgm_pick_ch(int D, int KL);
 int sgm_family_blocks(const RigDesc& g, int KL, int fam);
-int msum_blocks(const RigDesc& g, int side);
+int msum_blocks(const RigDesc& g, int side);   // side 0 row runs, 1 column runs, 2 tiles
+// per-device set-up of kernel attributes (dynamic LDS above 64 KB); call with
+// the device current.  0 on success.
+int kernels_setup(int device);
 void launch_census(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t);
 void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t);

@@ -16,6 +16,7 @@
 // cover many rigs of different sizes: block b belongs to rig i when
 // bs[i] <= b < bs[i+1].
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdlib>
 #include "rig.h"
 #include "kernels.h"
@@ -498,6 +499,35 @@ __global__ __launch_bounds__(256) void k_mask(const RigDesc* __restrict__ rigs, 
 constexpr int kRun = 256;   // pixels per block
 constexpr int kWinPad = 8;  // window entries before each partner window
 
+// acc[j] += pop(cc ^ window[base - d]) for lane q's disparities d = q + 64 j.
+// In the last group, lanes at or past Dp read as lane qlast = Dp - 1 - 64 (J-1)
+// (their sums are not stored) and lanes at or past D are zeroed by the caller.
+// Popcounts accumulate in place: v_bcnt_u32_b32 adds its second operand, one
+// instruction per 32-bit half (written out, since the compiler reassociates
+// pop + pop + acc into two plain popcounts and a three-way add).
+__device__ __forceinline__ unsigned bcnt_add(uint32_t x, unsigned acc)
+{
+    unsigned r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+
+template <int J>
+__device__ __forceinline__ void msum_add_frame(const uint64_t* L, uint64_t cc, int base, int q, int qlast,
+                                               unsigned (&acc)[J])
+{
+    uint64_t tv[J];
+    const uint64_t* w = L + base - q;       // w[-64 j] = window[base - d]
+#pragma unroll
+    for (int j = 0; j < J - 1; j++) tv[j] = w[-64 * j];
+    tv[J - 1] = L[base - qlast - 64 * (J - 1)];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const uint64_t x = cc ^ tv[j];
+        acc[j] = bcnt_add((uint32_t)(x >> 32), bcnt_add((uint32_t)x, acc[j]));
+    }
+}
+
 // SIDE = 0: frames 0/180 along row y; SIDE = 1: frames 90/270 along column x.
 // J = ceil(Dp / 64) groups of 64 disparities; every rig of the launch has this Dp.
 template <int SIDE, int J>
@@ -570,24 +600,13 @@ __global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ ri
     const int wave = tid >> 6, q = tid & 63;
     const int qlast = min(q, Dp - 1 - 64 * (J - 1));
     const bool store_last = q + 64 * (J - 1) < Dp;
-    const uint32_t mask_last = (q + 64 * (J - 1) < D) ? 0xFFFFFFFFu : 0u;
-    auto add_frame = [&](const uint64_t* L, uint64_t cc, int base, unsigned (&acc)[J]) {
-        uint64_t t[J];
-        const uint64_t* w = L + base - q;       // w[-64 j] = window[base - d]
-#pragma unroll
-        for (int j = 0; j < J - 1; j++) t[j] = w[-64 * j];
-        t[J - 1] = L[base - qlast - 64 * (J - 1)];
-#pragma unroll
-        for (int j = 0; j < J; j++) {
-            const unsigned v = (unsigned)__popcll(cc ^ t[j]);
-            acc[j] += (j == J - 1) ? (v & mask_last) : v;
-        }
-    };
+    const bool live_last = q + 64 * (J - 1) < D;
+    uint8_t* const msum = g.msum;
     for (int i = wave; i < kRun; i += 4) {
         const int pos = p0 + i;
         if (pos >= len) break;
         const int y = SIDE == 0 ? line : pos, x = SIDE == 0 ? pos : line;
-        gptr<uint8_t> o = G(g.msum) + ((size_t)y * Wp + x) * Dp;
+        gptr<uint8_t> o = G(msum) + ((size_t)y * Wp + x) * Dp;
         unsigned acc[J];
 #pragma unroll
         for (int j = 0; j < J; j++) acc[j] = 0;
@@ -597,11 +616,173 @@ __global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ ri
             if (store_last) acc[J - 1] = o[q + 64 * (J - 1)];
         }
         const int mb = __builtin_amdgcn_readfirstlane(LM[i]);   // one pixel per wave
-        if (mb & 1) add_frame(LA, LCa[i], (SIDE == 0 ? pos : Hp - 1 - pos) - loA, acc);
-        if (mb & 2) add_frame(LB, LCb[i], (SIDE == 0 ? Wp - 1 - pos : pos) - loB, acc);
+        if (mb & 1) msum_add_frame<J>(LA, LCa[i], (SIDE == 0 ? pos : Hp - 1 - pos) - loA, q, qlast, acc);
+        if (mb & 2) msum_add_frame<J>(LB, LCb[i], (SIDE == 0 ? Wp - 1 - pos : pos) - loB, q, qlast, acc);
 #pragma unroll
         for (int j = 0; j < J - 1; j++) o[q + 64 * j] = (uint8_t)acc[j];
-        if (store_last) o[q + 64 * (J - 1)] = (uint8_t)acc[J - 1];
+        if (store_last) o[q + 64 * (J - 1)] = live_last ? (uint8_t)acc[J - 1] : (uint8_t)0;
+    }
+}
+
+// Fused variant for cost sums over all four frames: a kMT x kMT tile of pixels
+// needs the partner windows of every frame line through it (tile rows for
+// frames 0/180, tile columns for frames 90/270: 4 kMT lines of kMT + D - 1
+// codes) and 4 kMT^2 centre codes, staged in LDS, and writes M once -- no
+// read-modify-write of M between two launches.  That is 118.5 KB of LDS at
+// D = 192, so one block per CU: the blocks are persistent and load the next
+// tile's windows into registers while the current tile is summed.  Block b
+// takes tiles s(b), s(b) + n, s(b) + 2n, ... (n blocks); s groups the blocks
+// that share an XCD (b mod 8 under round-robin dispatch) on consecutive
+// tiles, so neighbouring tiles, whose row windows overlap, share an L2.  The
+// two-launch form covers the disparity counts beyond what one block can hold.
+// Per pixel as above: one wave per pixel, lane q owns d = q + 64 j.
+// ---------------------------------------------------------------------------
+constexpr int kMT = 16;
+
+size_t msum_tile_lds(int D)
+{
+    return ((size_t)4 * kMT * (kMT + D - 1 + kWinPad) + 4 * kMT * kMT) * 8 + kMT * kMT;
+}
+
+template <int J>
+__global__ __launch_bounds__(1024) void k_msum_tile(const RigDesc* __restrict__ rigs, int nrig,
+                                                    const int* __restrict__ bs, int ntiles, int fuse)
+{
+    static_assert(kMT == 16, "one wave per line of each frame, one thread per tile pixel below 256");
+    // a line holds kWinPad + kMT + D - 1 <= 64 J + 23 entries: J + 1 chunks of 64
+    constexpr int NK = J + 1;
+    extern __shared__ uint64_t lds64[];
+    const int D = rigs[0].D, Dp = rigs[0].Dp;   // one disparity count per launch
+    const int SPAN = kMT + D - 1, LSTR = SPAN + kWinPad;
+    auto win = [&](int F, int l) { return lds64 + (F * kMT + l) * LSTR + kWinPad; };
+    uint64_t* CEN = lds64 + 4 * kMT * LSTR;     // [frame][pixel]
+    uint8_t* MB = (uint8_t*)(CEN + 4 * kMT * kMT);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int nb = gridDim.x, b = blockIdx.x;
+    int t = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+
+    struct Geo { int ri, x0, y0; };
+    auto geo = [&](int tt) __attribute__((always_inline)) {
+        Geo q;
+        q.ri = find_rig(bs, nrig, tt);
+        const int ntx = (rigs[q.ri].Wp + kMT - 1) / kMT, lb = tt - bs[q.ri];
+        q.x0 = (lb % ntx) * kMT;
+        q.y0 = (lb / ntx) * kMT;
+        return q;
+    };
+    // Frame F pixel of image pixel (y, x) and the frame column of window
+    // entry 0 of the line through the tile:
+    //   F0: row y,        col x,         window from x0 - (D-1)
+    //   F2: row y,        col W'-1-x,    window from W' - x0 - kMT - (D-1)
+    //   F1: row W'-1-x,   col H'-1-y,    window from H' - y0 - kMT - (D-1)
+    //   F3: row W'-1-x,   col y,         window from y0 - (D-1)
+    // pc: centre codes; pk: mask bytes, read only at stash time, so that
+    // nothing waits for the loads while the current tile is summed
+    uint64_t pw[4][NK], pc[4];
+    uint32_t pk[4];
+    auto load = [&](const Geo& q) __attribute__((always_inline)) {
+        const RigDesc& g = rigs[q.ri];
+        const int Wp = g.Wp, Hp = g.Hp;
+#pragma unroll
+        for (int F = 0; F < 4; F++) {
+            const int l = wave;
+            const bool rowf = (F & 1) == 0;
+            const bool line_ok = rowf ? q.y0 + l < Hp : q.x0 + l < Wp;
+            const int frow = rowf ? q.y0 + l : Wp - 1 - (q.x0 + l);
+            const int fcols = rowf ? Wp : Hp;
+            const int lo = F == 0 ? q.x0 - (D - 1)
+                         : F == 2 ? Wp - q.x0 - kMT - (D - 1)
+                         : F == 1 ? Hp - q.y0 - kMT - (D - 1) : q.y0 - (D - 1);
+            const bool use = ((fuse >> F) & 1) && line_ok;
+            const uint64_t* N = g.code[2 * F + 1] + (size_t)(use ? frow : 0) * fcols;
+#pragma unroll
+            for (int kk = 0; kk < NK; kk++) {
+                const int k = lane - kWinPad + 64 * kk, col = lo + k;
+                pw[F][kk] = (use && k >= 0 && k < SPAN && col >= 0 && col < fcols) ? G(N)[col] : 0;
+            }
+        }
+        if (tid < kMT * kMT) {
+            const int y = q.y0 + tid / kMT, x = q.x0 + tid % kMT;
+            const bool in = y < Hp && x < Wp;
+#pragma unroll
+            for (int F = 0; F < 4; F++) {
+                const bool rowf = (F & 1) == 0;
+                const int frow = rowf ? y : Wp - 1 - x;
+                const int fcol = F == 0 ? x : F == 2 ? Wp - 1 - x : F == 1 ? Hp - 1 - y : y;
+                const int fcols = rowf ? Wp : Hp;
+                // a set mask implies frame column >= D; the check keeps indices in range
+                const bool use = in && ((fuse >> F) & 1) && fcol >= D;
+                pc[F] = use ? G(g.code[2 * F])[(size_t)frow * fcols + fcol] : 0;
+                pk[F] = use ? G(g.mask[F])[(size_t)y * Wp + x] : 0;
+            }
+        }
+    };
+    auto stash = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int F = 0; F < 4; F++) {
+            uint64_t* W = win(F, wave);
+#pragma unroll
+            for (int kk = 0; kk < NK; kk++) {
+                const int k = lane - kWinPad + 64 * kk;
+                if (k < SPAN) W[k] = pw[F][kk];
+            }
+        }
+        if (tid < kMT * kMT) {
+            uint32_t pm = 0;
+#pragma unroll
+            for (int F = 0; F < 4; F++) {
+                CEN[F * kMT * kMT + tid] = pc[F];
+                pm |= (pk[F] != 0 ? 1u : 0u) << F;
+            }
+            MB[tid] = (uint8_t)pm;
+        }
+    };
+    const int q = lane;
+    const int qlast = min(q, Dp - 1 - 64 * (J - 1));
+    const bool store_last = q + 64 * (J - 1) < Dp;
+    const bool live_last = q + 64 * (J - 1) < D;
+    // wave w walks tile column w
+    auto sum_tile = [&](const Geo& gq) __attribute__((always_inline)) {
+        const RigDesc& g = rigs[gq.ri];
+        const int Wp = g.Wp, Hp = g.Hp;
+        uint8_t* const msum = g.msum;
+        const int tx = wave, x = gq.x0 + tx;
+        if (x >= Wp) return;
+        const uint32_t mcol = lane < kMT ? MB[lane * kMT + tx] : 0u;   // lane ty: mask bits of (ty, tx)
+        for (int ty = 0; ty < kMT; ty++) {
+            const int y = gq.y0 + ty;
+            if (y >= Hp) break;
+            const int i = ty * kMT + tx;
+            unsigned acc[J];
+#pragma unroll
+            for (int j = 0; j < J; j++) acc[j] = 0;
+            const int mb = __builtin_amdgcn_readlane(mcol, ty);   // one pixel per wave
+            if (mb & 1) msum_add_frame<J>(win(0, ty), CEN[i], tx + D - 1, q, qlast, acc);
+            if (mb & 4) msum_add_frame<J>(win(2, ty), CEN[2 * kMT * kMT + i], (kMT - 1 - tx) + D - 1, q, qlast, acc);
+            if (mb & 2) msum_add_frame<J>(win(1, tx), CEN[kMT * kMT + i], (kMT - 1 - ty) + D - 1, q, qlast, acc);
+            if (mb & 8) msum_add_frame<J>(win(3, tx), CEN[3 * kMT * kMT + i], ty + D - 1, q, qlast, acc);
+            gptr<uint8_t> o = G(msum) + ((size_t)y * Wp + x) * Dp;
+#pragma unroll
+            for (int j = 0; j < J - 1; j++) o[q + 64 * j] = (uint8_t)acc[j];
+            if (store_last) o[q + 64 * (J - 1)] = live_last ? (uint8_t)acc[J - 1] : (uint8_t)0;
+        }
+    };
+    Geo cur = geo(t);
+    load(cur);
+    for (;;) {
+        stash();
+        __syncthreads();
+        const int tn = t + nb;
+        Geo nxt = cur;
+        if (tn < ntiles) {
+            nxt = geo(tn);
+            load(nxt);          // in flight while this tile is summed
+        }
+        sum_tile(cur);
+        if (tn >= ntiles) break;
+        __syncthreads();        // LDS free for the next stash
+        t = tn;
+        cur = nxt;
     }
 }
 
@@ -1212,7 +1393,61 @@ size_t msum_lds(int D)
 
 int msum_blocks(const RigDesc& g, int side)
 {
+    if (side == 2) return ((g.Wp + kMT - 1) / kMT) * ((g.Hp + kMT - 1) / kMT);
     return side == 0 ? g.Hp * ((g.Wp + kRun - 1) / kRun) : g.Wp * ((g.Hp + kRun - 1) / kRun);
+}
+
+constexpr int kMsumTileMaxJ = 5;
+static size_t g_lds_per_block = 0;   // set by kernels_setup
+static int g_num_cu = 0;
+
+int kernels_setup(int device)
+{
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return -1;
+    const int lim = v;
+    if (hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return -1;
+    const void* fns[kMsumTileMaxJ] = {(const void*)k_msum_tile<1>, (const void*)k_msum_tile<2>,
+                                      (const void*)k_msum_tile<3>, (const void*)k_msum_tile<4>,
+                                      (const void*)k_msum_tile<5>};
+    for (const void* f : fns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
+            (void)hipGetLastError();
+            g_lds_per_block = 0;   // fused cost sum off: the two-launch form runs
+            return 0;
+        }
+    g_lds_per_block = (size_t)lim;
+    return 0;
+}
+
+// SISTER_MSUM_SPLIT=1 keeps the two-launch cost sum everywhere (comparison runs)
+static bool msum_split_forced()
+{
+    static const int v = [] {
+        const char* e = getenv("SISTER_MSUM_SPLIT");
+        return e && atoi(e) != 0 ? 1 : 0;
+    }();
+    return v != 0;
+}
+
+static bool msum_tile_fits(const DGroup& G)
+{
+    return (G.Dp + 63) / 64 <= kMsumTileMaxJ && msum_tile_lds(G.D) <= g_lds_per_block;
+}
+
+static void launch_msum_tile(const DGroup& G, int fuse, hipStream_t st)
+{
+    const size_t lds = msum_tile_lds(G.D);
+    const int ntiles = G.blocks[kGMsumTile];
+    const dim3 grid(std::min(ntiles, std::max(g_num_cu, 1))), block(1024);   // persistent, one per CU
+    const int* bs = G.bs[kGMsumTile];
+    switch ((G.Dp + 63) / 64) {
+    case 1: hipLaunchKernelGGL(k_msum_tile<1>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
+    case 2: hipLaunchKernelGGL(k_msum_tile<2>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
+    case 3: hipLaunchKernelGGL(k_msum_tile<3>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
+    case 4: hipLaunchKernelGGL(k_msum_tile<4>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
+    default: hipLaunchKernelGGL(k_msum_tile<5>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
+    }
 }
 
 template <int SIDE>
@@ -1238,6 +1473,10 @@ void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t)
     TB(kKMsum);
     for (int gi = 0; gi < ls.ngroups; gi++) {
         const DGroup& G = ls.groups[gi];
+        if ((fuse & 5) && (fuse & 10) && msum_tile_fits(G) && !msum_split_forced()) {
+            launch_msum_tile(G, fuse, st);                 // all frames, M written once
+            continue;
+        }
         if (fuse & 5) launch_msum_side<0>(G, fuse, st);    // frames 0/180 write M (zero when alone)
         if (fuse & 10) launch_msum_side<1>(G, fuse, st);   // frames 90/270 add (or write when alone)
     }

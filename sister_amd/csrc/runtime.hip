@@ -465,6 +465,7 @@ int sister_create(const sister_config* cfg, sister_ctx** out)
             ok = hipStreamCreateWithFlags(&f.side[j], hipStreamNonBlocking) == hipSuccess &&
                  hipEventCreateWithFlags(&f.join[j], hipEventDisableTiming) == hipSuccess;
     }
+    ok = ok && kernels_setup(c->device) == 0;
     if (!ok) {
         delete c;
         return SISTER_EDEVICE;
