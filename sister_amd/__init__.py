@@ -78,6 +78,7 @@ _SIGS = {
                                       C.POINTER(C.c_int), C.c_int]),
     "sister_debug_fetch": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, C.c_void_p, C.c_size_t]),
     "sister_debug_keep_sum": (C.c_int, [C.c_void_p, C.c_int]),
+    "sister_debug_vsweep": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 
@@ -254,6 +255,10 @@ class Context:
 
     def keep_sum(self, on=True):
         self._check(lib().sister_debug_keep_sum(self._h, 1 if on else 0))
+
+    def vsweep(self, mode):
+        """Vertical SGM families: 0 separate sweeps, 1 fused sweep, -1 automatic."""
+        self._check(lib().sister_debug_vsweep(self._h, int(mode)))
 
     def fetch(self, index, what, which, shape, dtype):
         a = np.empty(shape, dtype)

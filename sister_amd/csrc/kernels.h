@@ -19,6 +19,7 @@ enum GroupKernel {
     kGRows = 0, kGCols, kGDiag, kGAnti,   // SGM families
     kGMsumRows, kGMsumCols,               // cost sum: frames 0/180 (row runs), 90/270 (column runs)
     kGMsumTile,                           // cost sum: all frames, per 16x16 tile
+    kGVSweep,                             // fused vertical SGM families, per column block
     kNumGroupKernels
 };
 struct DGroup {
@@ -28,6 +29,8 @@ struct DGroup {
     const int* bs[kNumGroupKernels];   // block starts per kernel, nrig+1 entries each
     int blocks[kNumGroupKernels];
     int CH[4], KL[4];        // SGM: chunks of 16 disparities per lane, lanes per chain, per family
+    int vcols;               // fused vertical sweep: owned columns per block (0: separate families)
+    int maxHp;               // tallest padded rig of the group
 };
 constexpr int kMaxDGroups = 16;
 
@@ -47,7 +50,7 @@ struct LaunchSet {
 // individual kernels, for per-kernel event timing
 enum KernelId {
     kKCensus = 0, kKWta, kKMedianPrep, kKMedianSerial, kKMask, kKMsum,
-    kKRows, kKCols, kKDiag, kKAnti, kNumKernels
+    kKRows, kKCols, kKDiag, kKAnti, kKVSweep, kNumKernels
 };
 extern const char* const kKernelNames[kNumKernels];
 
@@ -63,6 +66,10 @@ int sgm_pick_kl(long cols_chains, int fam, int D);
 int sgm_pick_ch(int D, int KL);
 int sgm_family_blocks(const RigDesc& g, int KL, int fam);
 int msum_blocks(const RigDesc& g, int side);   // side 0 row runs, 1 column runs, 2 tiles
+// fused vertical sweep: owned columns per block for a group whose rigs have
+// `cols` padded columns in all and disparity count D (0: not used)
+int vsweep_pick_cols(long cols, int D, int choice);
+int vsweep_blocks(const RigDesc& g, int vcols);
 // per-device set-up of kernel attributes (dynamic LDS above 64 KB); call with
 // the device current.  0 on success.
 int kernels_setup(int device);

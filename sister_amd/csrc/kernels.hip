@@ -907,10 +907,33 @@ struct ChainGeo {
 };
 
 // one standard step over the lane's chunks.  m = cost pairs per chunk;
-// returns (L - M) pairs in tp.  trunc: 8-bit first-line truncation.
-template <int CH, int KL>
+// returns (L - M) pairs in tp (ACC: adds them to tp).  trunc: 8-bit
+// first-line truncation.
+template <int CH, int KL, bool ACC, class MF>
+__device__ __forceinline__ void lane_step_f(LaneState<CH>& st, const ChainGeo& cg, MF&& mpair,
+                                            uint32_t (&tp)[CH][8], bool trunc);
+
+template <int CH, int KL, bool ACC = false>
 __device__ __forceinline__ void lane_step(LaneState<CH>& st, const ChainGeo& cg, const uint32_t (&m)[CH][8],
                                           uint32_t (&tp)[CH][8], bool trunc)
+{
+    lane_step_f<CH, KL, ACC>(st, cg, [&](int q, int i) __attribute__((always_inline)) { return m[q][i]; }, tp, trunc);
+}
+
+// the same step with the cost pairs taken from packed bytes (4 per word, 16 per chunk)
+template <int CH, int KL, bool ACC = false>
+__device__ __forceinline__ void lane_step_pk(LaneState<CH>& st, const ChainGeo& cg, const uint4 (&mb)[CH],
+                                             uint32_t (&tp)[CH][8])
+{
+    lane_step_f<CH, KL, ACC>(st, cg, [&](int q, int i) __attribute__((always_inline)) {
+        const uint32_t w = i < 2 ? mb[q].x : i < 4 ? mb[q].y : i < 6 ? mb[q].z : mb[q].w;
+        return (i & 1) ? bytes23(w) : bytes01(w);
+    }, tp, false);
+}
+
+template <int CH, int KL, bool ACC, class MF>
+__device__ __forceinline__ void lane_step_f(LaneState<CH>& st, const ChainGeo& cg, MF&& mpair,
+                                            uint32_t (&tp)[CH][8], bool trunc)
 {
     const uint32_t nbl = row_shr1(st.L[8 * CH - 1], kGuard2);   // left lane's last pair (old)
     const uint32_t nbr = row_shl1(st.L[0], kGuard2);            // right lane's first pair (old)
@@ -934,11 +957,11 @@ __device__ __forceinline__ void lane_step(LaneState<CH>& st, const ChainGeo& cg,
             const us2 nb = umin2(left, right) + (us2){(unsigned short)kP1, (unsigned short)kP1};
             const us2 t = umin2(umin2(nb, U2(cur)), p2t);
             const us2 d = t - U2(mpp);
-            us2 l = d + U2(m[q][i]);
+            us2 l = d + U2(mpair(q, i));
             if (trunc) l = l & (us2){0xFF, 0xFF};
             const bool pad = !valid || (ht && i >= 4);
             l = pad ? U2(kGuard2) : l;
-            tp[q][i] = W2(d);
+            tp[q][i] = ACC ? W2(U2(tp[q][i]) + d) : W2(d);
             st.L[j] = W2(l);
             prv = cur;
             rmin = umin2(rmin, l);
@@ -1279,6 +1302,393 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 }
 
 // ---------------------------------------------------------------------------
+// K6b fused vertical families: the column, diagonal and anti-diagonal chains
+// of a pass in one sweep (same recurrence and pass rules as k_fam).  Lanes own
+// columns (kVKL = 4 lanes per column, CH chunks of 16 disparities each), so
+// every cell of a row sees its three vertical paths in one lane group:
+//   downward pass:  P = part[rows] + sum_f (L_f,down - M)  (u16, rows 1..H'-2)
+//   upward pass:    S = 8 M + P + sum_f (L_f,up - M), winner and sub-pixel
+// -- 8 bytes per cell over both passes, against 17 for the three separate
+// families (5 + 5 + 7).  Along a pass the diagonal-family state moves one
+// column per row (down-right, up-left) and the anti-diagonal state the other
+// way; both live in LDS, one slot per column, updated in place between
+// barriers.  Call FL the family whose state moves to higher columns (diagonal
+// going down, anti-diagonal going up) and FR the other.
+// A block owns ncol columns and also computes kVGhost ghost columns on each
+// side (one wave each: FL only on the left, FR only on the right, the only
+// families that flow inwards).  A wrong state entering at the ghost edge
+// moves inwards one column per row, so the owned columns stay exact for
+// kVGhost rows without any exchange with neighbouring blocks: one launch runs
+// one segment of kVGhost rows, and the states cross segments through memory
+// (owned columns stored at the end, owned + ghost columns loaded at the start).
+// No block ever waits for another.
+// Buffers: P lives in part[1..2] (u16 per cell), the segment states in part[3]
+// ([parity][cols, diag, anti][W'][Dp/2 + 4] words: pairs, then minL').
+// ---------------------------------------------------------------------------
+constexpr int kVGhost = 16;        // ghost columns per side = rows per launch
+constexpr int kVKL = 4;            // lanes per column
+constexpr int kVMaxThreads = 768;  // 12 waves: up to 168 VGPRs per lane
+
+template <int CH>
+__device__ __forceinline__ void vs_guard_all(LaneState<CH>& st)
+{
+    static_for<0, 8 * CH>([&](auto jc) { st.L[decltype(jc)::value] = kGuard2; });
+    st.mp = 0;
+}
+
+// S pair j (0 <= j < 8 CH) of the lane, by a select tree
+template <int CH>
+__device__ __forceinline__ uint32_t vs_pick(const uint32_t (&S)[CH][8], int j)
+{
+    uint32_t r = 0;
+    static_for<0, CH>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int i = j - 8 * q;
+        const uint32_t a0 = (i & 1) ? S[q][1] : S[q][0], a1 = (i & 1) ? S[q][3] : S[q][2];
+        const uint32_t a2 = (i & 1) ? S[q][5] : S[q][4], a3 = (i & 1) ? S[q][7] : S[q][6];
+        const uint32_t b0 = (i & 2) ? a1 : a0, b1 = (i & 2) ? a3 : a2;
+        const uint32_t v = (i & 4) ? b1 : b0;
+        r = (i >= 0 && i < 8) ? v : r;
+    });
+    return r;
+}
+
+template <int KL>
+__device__ __forceinline__ uint32_t group_max(uint32_t v)
+{
+    return ~group_min<KL>(~v);
+}
+
+template <int CH, int DIR, bool KEEP, bool FULL>
+__global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restrict__ rigs, int nrig,
+                                                        const int* __restrict__ bs, int ncol, int seg, int mode)
+{
+    // KEEP: also the debug sums S (all rows) and the edge rows they need.
+    // FULL: D == Dp == 64 CH, every chunk of every lane is real (no padding
+    // chunk, no half chunk), so no per-chunk lane conditions are needed.
+    constexpr bool keep_sum = KEEP;
+    constexpr int KL = kVKL, NP = 8 * CH, SWL = KL * NP + 4;
+    extern __shared__ uint32_t xs[];               // [FL, FR][nt][SWL]
+    const int ri = find_rig(bs, nrig, blockIdx.x);
+    const RigDesc& g = rigs[ri];
+    const int Wp = g.Wp, Hp = g.Hp, D = g.D, Dp = g.Dp;
+    const int t0 = seg * kVGhost;
+    if (t0 >= Hp) return;                          // the rig has no such segment: whole block
+    const int nrows = min(kVGhost, Hp - t0);
+    const int nt = ncol + 2 * kVGhost;
+    const int tid = threadIdx.x, c = tid / KL;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = (int)(blockDim.x >> 6);
+    const int x = (blockIdx.x - bs[ri]) * ncol - kVGhost + c;
+    const bool act = x >= 0 && x < Wp;
+    const bool lghost = wave == 0, rghost = wave == nwaves - 1, own = !lghost && !rghost;
+    const bool doFL = !rghost, doFR = !lghost;
+    ChainGeo cg;
+    cg.lid = tid % KL;
+    cg.nq = FULL ? KL * CH : Dp >> 4;
+    cg.half_tail = FULL ? false : (D & 15) != 0;
+    const int boff = cg.lid * CH * 16;
+    uint32_t* XL = xs;
+    uint32_t* XR = xs + nt * SWL;
+    constexpr int famL = DIR == 0 ? 2 : 3, famR = DIR == 0 ? 3 : 2;   // family ids of FL, FR
+    const int SW = Dp / 2 + 4;
+    uint32_t* const Sg = (uint32_t*)g.part[3];
+    uint16_t* const Pd = (uint16_t*)g.part[1];
+    const uint8_t* const msum = g.msum;
+    auto sg_col = [&](int par, int f) { return Sg + ((size_t)(par * 3 + f - 1) * Wp + x) * SW; };
+
+    // lane state <-> a column vector in memory (pairs, then minL' at word Dp/2)
+    auto st_load = [&](LaneState<CH>& st, const uint32_t* v) __attribute__((always_inline)) {
+        static_for<0, CH>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int qg = cg.lid * CH + q;
+            uint4 a = make_uint4(kGuard2, kGuard2, kGuard2, kGuard2), b = a;
+            if (act && (FULL || qg < cg.nq)) {
+                a = LD16(v + qg * 8);
+                b = LD16(v + qg * 8 + 4);
+            }
+            st.L[8 * q + 0] = a.x; st.L[8 * q + 1] = a.y; st.L[8 * q + 2] = a.z; st.L[8 * q + 3] = a.w;
+            st.L[8 * q + 4] = b.x; st.L[8 * q + 5] = b.y; st.L[8 * q + 6] = b.z; st.L[8 * q + 7] = b.w;
+        });
+        st.mp = act ? G(v)[Dp / 2] : 0u;
+    };
+    auto st_store = [&](const LaneState<CH>& st, uint32_t* v) __attribute__((always_inline)) {
+        static_for<0, CH>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int qg = cg.lid * CH + q;
+            if (FULL || qg < cg.nq) {
+                ST16(v + qg * 8, make_uint4(st.L[8 * q + 0], st.L[8 * q + 1], st.L[8 * q + 2], st.L[8 * q + 3]));
+                ST16(v + qg * 8 + 4, make_uint4(st.L[8 * q + 4], st.L[8 * q + 5], st.L[8 * q + 6], st.L[8 * q + 7]));
+            }
+        });
+        if (cg.lid == 0) G(v)[Dp / 2] = st.mp;
+    };
+    // LDS slots
+    auto slot_write = [&](uint32_t* X, const LaneState<CH>& st) __attribute__((always_inline)) {
+        uint32_t* w = X + c * SWL + cg.lid * NP;
+        static_for<0, 2 * CH>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            *(uint4*)(w + 4 * k) = make_uint4(st.L[4 * k], st.L[4 * k + 1], st.L[4 * k + 2], st.L[4 * k + 3]);
+        });
+        if (cg.lid == 0) X[c * SWL + KL * NP] = st.mp;
+    };
+    auto slot_read = [&](const uint32_t* X, int cc, LaneState<CH>& st) __attribute__((always_inline)) {
+        if (cc < 0 || cc >= nt) {   // beyond the computed columns: a border state
+            vs_guard_all<CH>(st);
+            return;
+        }
+        const uint32_t* w = X + cc * SWL + cg.lid * NP;
+        static_for<0, 2 * CH>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const uint4 v = *(const uint4*)(w + 4 * k);
+            st.L[4 * k] = v.x; st.L[4 * k + 1] = v.y; st.L[4 * k + 2] = v.z; st.L[4 * k + 3] = v.w;
+        });
+        st.mp = X[cc * SWL + KL * NP];
+    };
+    auto row_of = [&](int t) { return DIR == 0 ? t : Hp - 1 - t; };
+    auto load_m = [&](int t, uint4 (&b)[CH]) __attribute__((always_inline)) {
+        const size_t cell = ((size_t)row_of(t) * Wp + x) * Dp + boff;
+        static_for<0, CH>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const bool ok = act && (FULL || cg.lid * CH + q < cg.nq);
+            b[q] = ok ? LD16(msum + cell + 16 * q) : make_uint4(0, 0, 0, 0);
+        });
+    };
+    // exact L of a family on an edge row (debug sums only), as k_fam stores it
+    auto edge_out = [&](int f, int y, const LaneState<CH>& st) __attribute__((always_inline)) {
+        static_for<0, CH>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int qg = cg.lid * CH + q;
+            if (qg >= cg.nq) return;
+            uint16_t* e = g.edge + ((size_t)(f * 2 + (y == 0 ? 0 : 1)) * Wp + x) * Dp + qg * 16;
+            const uint4 a = make_uint4(st.L[8 * q + 0], st.L[8 * q + 1], st.L[8 * q + 2], st.L[8 * q + 3]);
+            uint4 bb = make_uint4(st.L[8 * q + 4], st.L[8 * q + 5], st.L[8 * q + 6], st.L[8 * q + 7]);
+            if (cg.half_tail && qg == cg.nq - 1) bb = make_uint4(0, 0, 0, 0);
+            ST16(e, a);
+            ST16(e + 8, bb);
+        });
+    };
+    // S of an edge row (debug): the four families' exact rows summed
+    auto edge_sum = [&](int y) __attribute__((always_inline)) {
+        const int e = y == 0 ? 0 : 1;
+        static_for<0, CH>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int qg = cg.lid * CH + q;
+            if (qg >= cg.nq) return;
+            const size_t eo = (size_t)x * Dp + qg * 16;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                uint4 acc = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int fm = 0; fm < 4; fm++)
+                    acc = add_u16x8(acc, LD16(g.edge + (size_t)(fm * 2 + e) * Wp * Dp + eo + 8 * h));
+                ST16(g.sum + (size_t)y * Wp * Dp + eo + 8 * h, acc);
+            }
+        });
+    };
+
+    const bool want_sub = DIR == 1 && mode == 0 && g.subpix != nullptr;
+    uint16_t* const outm = g.out[mode];
+    LaneState<CH> sc, sx;
+    uint4 mb[CH];
+    if (t0 > 0) {
+        const int pin = (seg - 1) & 1;
+        if (own) st_load(sc, sg_col(pin, 1));
+        if (doFL) {
+            st_load(sx, sg_col(pin, famL));
+            slot_write(XL, sx);
+        }
+        if (doFR) {
+            st_load(sx, sg_col(pin, famR));
+            slot_write(XR, sx);
+        }
+    }
+    load_m(t0, mb);
+    __syncthreads();
+#pragma unroll 1
+    for (int i = 0; i < nrows; i++) {
+        const int t = t0 + i, y = row_of(t);
+        uint32_t acc[CH][8];
+        uint4 mc[CH];
+        static_for<0, CH>([&](auto qc) { mc[decltype(qc)::value] = mb[decltype(qc)::value]; });
+        if (i + 1 < nrows) load_m(t + 1, mb);
+        if (t == 0) {
+            // first line of the pass: every chain takes c~ and adds nothing (sgm.cpp:103-207)
+            uint32_t m[CH][8];
+            static_for<0, CH>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                unpack16(mc[q], m[q]);
+#pragma unroll
+                for (int k = 0; k < 8; k++) m[q][k] = tilde2(m[q][k]);
+            });
+            if (own) lane_set<CH, KL>(sc, cg, m, act);
+            lane_set<CH, KL>(sx, cg, m, act);
+            if (doFL) slot_write(XL, sx);
+            if (doFR) slot_write(XR, sx);
+            if (DIR == 1 && keep_sum && g.sum && own && act) edge_sum(y);
+            __syncthreads();
+            continue;
+        }
+        const bool interior = y >= 1 && y <= Hp - 2;
+        const bool edge_row = !interior;
+        const size_t cell = ((size_t)y * Wp + x) * Dp + boff;
+        // addends of the row's output, loaded now and used after the three steps
+        uint4 ob[2 * CH];
+        static_for<0, 2 * CH>([&](auto kc) { ob[decltype(kc)::value] = make_uint4(0, 0, 0, 0); });
+        if (own && act && interior) {
+            static_for<0, CH>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if (!FULL && cg.lid * CH + q >= cg.nq) return;
+                if (DIR == 0) {
+                    ob[q] = LD16(g.part[0] + cell + 16 * q);
+                } else {
+                    ob[2 * q] = LD16(Pd + cell + 16 * q);
+                    ob[2 * q + 1] = LD16(Pd + cell + 16 * q + 8);
+                }
+            });
+        }
+        // (the steps' L - M outputs are not kept: their sum is formed after the
+        // three steps from the new states, sum_f (L_f - M) = sum_f L_f - 3 M,
+        // which keeps 24 registers free while the states are stepped)
+        uint32_t dead[CH][8];
+        if (own) {
+            lane_step_pk<CH, KL>(sc, cg, mc, dead);
+            if (edge_row && keep_sum && g.sum && act) edge_out(1, y, sc);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (doFL) slot_read(XL, c - 1, sx);
+        __syncthreads();
+        if (doFL) {
+            lane_step_pk<CH, KL>(sx, cg, mc, dead);
+            lane_guard<CH>(sx, act);
+            slot_write(XL, sx);
+            if (own && edge_row && keep_sum && g.sum && act) edge_out(famL, y, sx);
+        }
+        if (doFR) slot_read(XR, c + 1, sx);
+        __syncthreads();
+        if (doFR) {
+            lane_step_pk<CH, KL>(sx, cg, mc, dead);
+            lane_guard<CH>(sx, act);
+            slot_write(XR, sx);
+            if (own && edge_row && keep_sum && g.sum && act) edge_out(famR, y, sx);
+        }
+        __syncthreads();
+        if (!own || !act) continue;
+        if (edge_row) {
+            if (DIR == 1 && keep_sum && g.sum) edge_sum(y);   // y == 0
+            continue;
+        }
+        // acc = (L_cols + L_FL + L_FR) - 3 M from the new states of this column,
+        // one chunk at a time (scheduling barriers keep the compiler from
+        // hoisting every chunk's LDS reads at once)
+        auto chunk_sum = [&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            const uint32_t* wl = XL + c * SWL + cg.lid * NP + 8 * q;
+            const uint32_t* wr = XR + c * SWL + cg.lid * NP + 8 * q;
+            const uint4 l0 = *(const uint4*)wl, l1 = *(const uint4*)(wl + 4);
+            const uint4 r0 = *(const uint4*)wr, r1 = *(const uint4*)(wr + 4);
+            const uint32_t lv[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+            const uint32_t rv[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+            uint32_t mq[8];
+            unpack16(mc[q], mq);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const us2 m3 = U2(mq[k]) * (us2){3, 3};
+                acc[q][k] = W2(U2(sc.L[8 * q + k]) + U2(lv[k]) + U2(rv[k]) - m3);
+            }
+        };
+        if (DIR == 0) {
+            static_for<0, CH>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if (!FULL && cg.lid * CH + q >= cg.nq) return;
+                chunk_sum(qc);
+                uint32_t pr[8];
+                unpack16(ob[q], pr);
+#pragma unroll
+                for (int k = 0; k < 8; k++) acc[q][k] = W2(U2(acc[q][k]) + U2(pr[k]));
+                ST16(Pd + cell + 16 * q, make_uint4(acc[q][0], acc[q][1], acc[q][2], acc[q][3]));
+                ST16(Pd + cell + 16 * q + 8, make_uint4(acc[q][4], acc[q][5], acc[q][6], acc[q][7]));
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            continue;
+        }
+        // upward pass: S and the winner (hpp:283), as k_fam's final sweep
+        const int oy = y - D, ox = x - D;
+        const bool crop = oy >= 0 && oy < g.H && ox >= 0 && ox < g.W;
+        if (!(crop || keep_sum)) continue;
+        uint32_t key = 0xFFFFFFFFu;
+        static_for<0, CH>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int qg = cg.lid * CH + q;
+            chunk_sum(qc);
+            const uint32_t pw[8] = {ob[2 * q].x, ob[2 * q].y, ob[2 * q].z, ob[2 * q].w,
+                                    ob[2 * q + 1].x, ob[2 * q + 1].y, ob[2 * q + 1].z, ob[2 * q + 1].w};
+            uint32_t mq[8];
+            unpack16(mc[q], mq);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const us2 s8 = U2(mq[k]) << (us2){3, 3};
+                acc[q][k] = W2(s8 + U2(pw[k]) + U2(acc[q][k]));   // S pairs
+            }
+            if (!FULL && qg >= cg.nq) return;
+            if (keep_sum && g.sum) {
+                uint16_t* sp = g.sum + cell + 16 * q;
+                ST16(sp, make_uint4(acc[q][0], acc[q][1], acc[q][2], acc[q][3]));
+                ST16(sp + 8, make_uint4(acc[q][4], acc[q][5], acc[q][6], acc[q][7]));
+            }
+            // keys (S << 9) | (d - the lane's first d): compile-time low bits, so
+            // no per-pair lane constants are kept across rows; the lane's base
+            // is added after the minimum (d < 512, no carry into S).  In the
+            // half-filled last chunk (D % 16 == 8) pairs 4..7 are padding.
+            const bool tail = cg.half_tail && qg == cg.nq - 1;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint32_t k0 = ((acc[q][k] & 0xFFFFu) << 9) | (uint32_t)(16 * q + 2 * k);
+                const uint32_t k1 = ((acc[q][k] >> 16) << 9) | (uint32_t)(16 * q + 2 * k + 1);
+                const uint32_t kk = min(k0, k1);
+                key = (k < 4 || !tail) ? min(key, kk) : key;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        if (!crop) continue;
+        if (key != 0xFFFFFFFFu) key += (uint32_t)(cg.lid * CH * 16);   // lanes of padding only keep ~0
+        key = group_min<KL>(key);   // the four lanes of a column share crop
+        const uint32_t best = key & 511u;
+        float delta = 0.0f;
+        if (want_sub) {
+            // S at best - 1 and best + 1 from the lanes holding them
+            auto held = [&](int d) __attribute__((always_inline)) {
+                const int j = (d >> 1) - cg.lid * NP;
+                const uint32_t w = vs_pick<CH>(acc, j);
+                const uint32_t v = (d & 1) ? (w >> 16) : (w & 0xFFFFu);
+                return (d >= 0 && j >= 0 && j < NP) ? v : 0u;
+            };
+            const int sm = (int)group_max<KL>(held((int)best - 1));
+            const int sp = (int)group_max<KL>(held((int)best + 1));
+            if (best > 0 && (int)best < D - 1) {
+                const int s0 = (int)(key >> 9);
+                const int den = sm - 2 * s0 + sp;
+                if (den > 0) delta = (float)(sm - sp) / (2.0f * (float)den);
+            }
+        }
+        if (cg.lid == 0) {
+            const size_t o = (size_t)oy * g.W + ox;
+            if (outm) G(outm)[o] = (uint16_t)min(best * 255u, 65535u);
+            if (mode == 0 && g.disp) G(g.disp)[o] = (uint16_t)best;
+            if (want_sub) G(g.subpix)[o] = (float)best + delta;
+        }
+    }
+    // states at the end of the segment, for the next launch
+    if (t0 + nrows < Hp && own && act) {
+        const int pout = seg & 1;
+        st_store(sc, sg_col(pout, 1));
+        slot_read(XL, c, sx);
+        st_store(sx, sg_col(pout, famL));
+        slot_read(XR, c, sx);
+        st_store(sx, sg_col(pout, famR));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
 // SGM launches: (CH chunks per lane, KL lanes per chain) instantiations
@@ -1344,7 +1754,7 @@ int sgm_family_blocks(const RigDesc& g, int KL, int fam)
 
 const char* const kKernelNames[kNumKernels] = {
     "census", "wta", "median_prep", "median_serial", "mask", "msum",
-    "sgm_rows", "sgm_cols", "sgm_diag", "sgm_anti",
+    "sgm_rows", "sgm_cols", "sgm_diag", "sgm_anti", "sgm_vsweep",
 };
 
 #define TB(id) do { if (t) t->begin(id, st); } while (0)
@@ -1417,6 +1827,18 @@ int kernels_setup(int device)
             return 0;
         }
     g_lds_per_block = (size_t)lim;
+#define SISTER_VS_FNS(K, F)                                                                         \
+    (const void*)k_vsweep<1, 0, K, F>, (const void*)k_vsweep<2, 0, K, F>, (const void*)k_vsweep<3, 0, K, F>, \
+        (const void*)k_vsweep<4, 0, K, F>, (const void*)k_vsweep<1, 1, K, F>, (const void*)k_vsweep<2, 1, K, F>, \
+        (const void*)k_vsweep<3, 1, K, F>, (const void*)k_vsweep<4, 1, K, F>
+    const void* vfns[24] = {SISTER_VS_FNS(false, false), SISTER_VS_FNS(false, true), SISTER_VS_FNS(true, false)};
+#undef SISTER_VS_FNS
+    for (const void* f : vfns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
+            (void)hipGetLastError();
+            g_lds_per_block = 0;
+            return 0;
+        }
     return 0;
 }
 
@@ -1483,8 +1905,97 @@ void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t)
     TE(kKMsum);
 }
 
+static int vs_ch(int D) { return ((D + 15) / 16 + kVKL - 1) / kVKL; }
+
+static size_t vs_lds(int D, int vcols)
+{
+    return (size_t)2 * (vcols + 2 * kVGhost) * (32 * vs_ch(D) + 4) * sizeof(uint32_t);
+}
+
+int vsweep_pick_cols(long cols, int D, int choice)
+{
+    // choice: 0 separate families, 1 fused sweep where D allows, -1 automatic;
+    // SISTER_VSWEEP=0/1 in the environment overrides the automatic choice
+    static const int env = [] {
+        const char* e = getenv("SISTER_VSWEEP");
+        return e ? atoi(e) : -1;
+    }();
+    const int forced = choice >= 0 ? choice : env;
+    // up to 192 disparities (3 chunks per lane): beyond, the state spills
+    if (forced == 0 || vs_ch(D) > 3) return 0;
+    // 144 owned columns: 176 computed, 11 waves; 16 blocks per 2304-wide rig
+    const int vcols = 144;
+    if (vs_lds(D, vcols) > g_lds_per_block) return 0;
+    // enough blocks to fill the chip (few rigs keep the per-chain families,
+    // which spread a single rig over every CU)
+    if (forced != 1 && (cols + vcols - 1) / vcols < std::max(g_num_cu / 2, 1)) return 0;
+    return vcols;
+}
+
+int vsweep_blocks(const RigDesc& g, int vcols)
+{
+    return vcols > 0 ? (g.Wp + vcols - 1) / vcols : 0;
+}
+
+static void launch_vsweep(const DGroup& G, int mode, int keep_sum, hipStream_t st)
+{
+    const int nt = G.vcols + 2 * kVGhost;
+    const size_t lds = vs_lds(G.D, G.vcols);
+    const dim3 grid(G.blocks[kGVSweep]), block(kVKL * nt);
+    const int* bs = G.bs[kGVSweep];
+    const int nseg = (G.maxHp + kVGhost - 1) / kVGhost;
+    const int ch = vs_ch(G.D);
+    const bool full = G.D == G.Dp && G.Dp == 64 * ch;
+    for (int dir = 0; dir < 2; dir++)
+        for (int seg = 0; seg < nseg; seg++) {
+#define SISTER_VS(CHV, DIRV)                                                                                \
+    do {                                                                                                    \
+        if (keep_sum)                                                                                       \
+            hipLaunchKernelGGL((k_vsweep<CHV, DIRV, true, false>), grid, block, lds, st, G.rigs, G.nrig, bs, \
+                               G.vcols, seg, mode);                                                         \
+        else if (full)                                                                                      \
+            hipLaunchKernelGGL((k_vsweep<CHV, DIRV, false, true>), grid, block, lds, st, G.rigs, G.nrig, bs, \
+                               G.vcols, seg, mode);                                                         \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_vsweep<CHV, DIRV, false, false>), grid, block, lds, st, G.rigs, G.nrig,   \
+                               bs, G.vcols, seg, mode);                                                     \
+    } while (0)
+            if (dir == 0) {
+                switch (ch) {
+                case 1: SISTER_VS(1, 0); break;
+                case 2: SISTER_VS(2, 0); break;
+                case 3: SISTER_VS(3, 0); break;
+                default: SISTER_VS(4, 0); break;
+                }
+            } else {
+                switch (ch) {
+                case 1: SISTER_VS(1, 1); break;
+                case 2: SISTER_VS(2, 1); break;
+                case 3: SISTER_VS(3, 1); break;
+                default: SISTER_VS(4, 1); break;
+                }
+            }
+#undef SISTER_VS
+        }
+}
+
 void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, const FamStreams* fs, KernelTimer* t)
 {
+    bool all_fused = true;
+    for (int gi = 0; gi < ls.ngroups; gi++) all_fused = all_fused && ls.groups[gi].vcols > 0;
+    if (all_fused) {
+        // rows family, then the fused vertical sweep (which reads part[rows])
+        for (int gi = 0; gi < ls.ngroups; gi++) {
+            const DGroup& G = ls.groups[gi];
+            TB_S(kKRows, st);
+            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
+            TE_S(kKRows, st);
+            TB_S(kKVSweep, st);
+            launch_vsweep(G, mode, keep_sum, st);
+            TE_S(kKVSweep, st);
+        }
+        return;
+    }
     // rows on st, columns and diagonals on the side streams, joined before
     // the anti-diagonal family
     hipStream_t sc = st, sd = st;
@@ -1497,6 +2008,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
     }
     for (int gi = 0; gi < ls.ngroups; gi++) {
         const DGroup& G = ls.groups[gi];
+        if (G.vcols > 0) continue;
         TB_S(kKRows, st);
         SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
         TE_S(kKRows, st);
@@ -1515,6 +2027,15 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
     }
     for (int gi = 0; gi < ls.ngroups; gi++) {
         const DGroup& G = ls.groups[gi];
+        if (G.vcols > 0) {
+            TB_S(kKRows, st);
+            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
+            TE_S(kKRows, st);
+            TB_S(kKVSweep, st);
+            launch_vsweep(G, mode, keep_sum, st);
+            TE_S(kKVSweep, st);
+            continue;
+        }
         const size_t lds = (size_t)64 * G.CH[3] * 32;
         TB_S(kKAnti, st);
         SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[3], mode, keep_sum);

@@ -176,6 +176,7 @@ struct sister_ctx {
     EventTimer timer;
     bool profiling = false;
     bool keep_sum = false;
+    int vsweep = -1;              // vertical SGM families: -1 automatic, 0 separate, 1 fused
     Plan last;                 // plan of the last sister_run_device (kept for debug fetch)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -295,17 +296,19 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         p.host_rigs.push_back(g);
     }
     // SGM groups: runs of equal D
-    struct GroupHost { int i0, i1, KL[4], CH[4]; };
+    struct GroupHost { int i0, i1, KL[4], CH[4], vcols, maxHp; };
     std::vector<GroupHost> groups;
     for (int k = 0; k < n;) {
         int e = k;
         long chains = 0;
         while (e < n && p.host_rigs[e].D == p.host_rigs[k].D) chains += p.host_rigs[e++].Wp;
-        GroupHost G{k, e, {}, {}};
+        GroupHost G{k, e, {}, {}, 0, 0};
         for (int f = 0; f < 4; f++) {
             G.KL[f] = sgm_pick_kl(chains, f, p.host_rigs[k].D);
             G.CH[f] = sgm_pick_ch(p.host_rigs[k].D, G.KL[f]);
         }
+        G.vcols = vsweep_pick_cols(chains, p.host_rigs[k].D, c->vsweep);
+        for (int j = k; j < e; j++) G.maxHp = std::max(G.maxHp, p.host_rigs[j].Hp);
         groups.push_back(G);
         k = e;
     }
@@ -350,7 +353,8 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
             const RigDesc& g = p.host_rigs[G.i0 + k];
             for (int f = 0; f < kNumGroupKernels; f++) {
                 gbs[gpos + f * (ng + 1) + k] = tot[f];
-                tot[f] += f < 4 ? sgm_family_blocks(g, G.KL[f], f) : msum_blocks(g, f - kGMsumRows);
+                tot[f] += f < 4 ? sgm_family_blocks(g, G.KL[f], f)
+                        : f == kGVSweep ? vsweep_blocks(g, G.vcols) : msum_blocks(g, f - kGMsumRows);
             }
         }
         for (int f = 0; f < kNumGroupKernels; f++) gbs[gpos + f * (ng + 1) + ng] = tot[f];
@@ -387,6 +391,8 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
             sg.KL[f] = G.KL[f];
             sg.CH[f] = G.CH[f];
         }
+        sg.vcols = G.vcols;
+        sg.maxHp = G.maxHp;
     }
     return SISTER_OK;
 }
@@ -722,6 +728,14 @@ int sister_debug_keep_sum(sister_ctx* c, int on)
 {
     if (!c) return SISTER_EINVAL;
     c->keep_sum = on != 0;
+    return SISTER_OK;
+}
+
+// debug: form of the vertical SGM families in plans built from now on
+int sister_debug_vsweep(sister_ctx* c, int mode)
+{
+    if (!c || mode < -1 || mode > 1) return SISTER_EINVAL;
+    c->vsweep = mode;
     return SISTER_OK;
 }
 

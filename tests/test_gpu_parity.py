@@ -72,7 +72,39 @@ def _compare_stages(ctx, O, W, H, D, seed, modes=7):
 @pytest.mark.parametrize("W,H,D,seed", SMALL)
 def test_stages_bit_exact(gpu_ctx, oracle_mod, W, H, D, seed):
     print(f"[parity] W={W} H={H} D={D} seed={seed}", flush=True)
-    bad = _compare_stages(gpu_ctx, oracle_mod, W, H, D, seed)
+    gpu_ctx.vsweep(0)
+    try:
+        bad = _compare_stages(gpu_ctx, oracle_mod, W, H, D, seed)
+    finally:
+        gpu_ctx.vsweep(-1)
+    assert not bad, bad
+
+
+# the fused vertical sweep (forced on at sizes where it would not be chosen):
+# one block (144 columns) and several, segment boundaries (16 rows) inside and
+# at the end, partial last blocks, every disparity layout up to 192
+FUSED = [
+    (64, 48, 16, 3),
+    (96, 64, 24, 4),
+    (128, 96, 32, 5),
+    (72, 40, 64, 6),
+    (80, 60, 72, 7),
+    (160, 120, 40, 8),
+    (40, 36, 8, 9),
+    (200, 44, 56, 10),
+    (96, 52, 120, 11),
+    (64, 40, 192, 12),
+]
+
+
+@pytest.mark.parametrize("W,H,D,seed", FUSED)
+def test_stages_bit_exact_fused_vertical(gpu_ctx, oracle_mod, W, H, D, seed):
+    print(f"[parity fused] W={W} H={H} D={D} seed={seed}", flush=True)
+    gpu_ctx.vsweep(1)
+    try:
+        bad = _compare_stages(gpu_ctx, oracle_mod, W, H, D, seed)
+    finally:
+        gpu_ctx.vsweep(-1)
     assert not bad, bad
 
 
