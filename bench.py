@@ -40,16 +40,21 @@ HBM_PEAK = 8.0e12
 # SGM path is 8 (+P1, min d-1, min d+1, min P2 term, -minL, +C, running min,
 # +S) and each chain-family kernel sweeps two paths; the final argmin (1) is
 # fused into sgm_anti.  Total 28 + 4 + 3*16 + 17 = 97.
-OPS_PER_CELL = {"wta": 28, "msum": 4, "sgm_rows": 16, "sgm_cols": 16, "sgm_diag": 16, "sgm_anti": 17}
+OPS_PER_CELL = {"wta": 28, "msum": 4, "sgm_rows": 16, "sgm_cols": 16, "sgm_diag": 16, "sgm_anti": 17,
+                "sgm_vsweep": 49}   # the fused vertical sweep: 3 families x 2 sweeps x 8, + argmin
 
 # Algorithmic HBM bytes per padded cell of the HBM-bound kernels.  Each SGM
 # chain family reads the u8 cost M in both sweeps (2), writes its forward
 # (L - M) byte (1), reads it back and writes the two-direction sum (2): 5.
 # The anti-diagonal family, which forms S and the winner, instead reads the
 # three other families' bytes (3) and writes nothing in its backward sweep: 7.
-# The cost sum writes M once from the frame-0/180 pass (1) and reads and
-# rewrites it in the frame-90/270 pass (2): 3.
-BYTES_PER_CELL = {"sgm_rows": 5, "sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "msum": 3}
+# The fused vertical sweep (columns, diagonals, anti-diagonals together)
+# reads M (1) and the rows family's byte (1) and writes the u16 partial sum P
+# (2) going down, then reads M (1) and P (2) going up: 7 (against 5 + 5 + 7
+# for the three separate families).  The cost sum over 16x16 tiles writes M
+# once (1); its two-launch form (frames 0/180, then 90/270 read-modify-write)
+# moves 3.
+BYTES_PER_CELL = {"sgm_rows": 5, "sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "sgm_vsweep": 7, "msum": 1}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch16_rig1080_d192.json")
 
 
@@ -202,6 +207,11 @@ def run(args):
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                     "bytes_per_cell": BYTES_PER_CELL[dom], "launches_per_step": launches[dom]}
+        if dom in OPS_PER_CELL:
+            # the same kernel priced by its algorithmic integer work
+            v = OPS_PER_CELL[dom] * cells / (kern[dom] * 1e-3)
+            roofline["valu"] = {"achieved": v / 1e12, "peak": VALU_PEAK / 1e12, "unit": "T int-lane-ops/s",
+                                "frac": v / VALU_PEAK, "ops_per_cell": OPS_PER_CELL[dom]}
     else:
         achieved = OPS_PER_CELL.get(dom, 0) * cells / (kern[dom] * 1e-3)
         roofline = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,

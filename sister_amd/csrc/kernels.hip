@@ -1326,6 +1326,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 // ([parity][cols, diag, anti][W'][Dp/2 + 4] words: pairs, then minL').
 // ---------------------------------------------------------------------------
 constexpr int kVGhost = 16;        // ghost columns per side = rows per launch
+constexpr int kVCols = 144;        // owned columns per block (16 blocks per 2304-wide rig)
 constexpr int kVKL = 4;            // lanes per column
 constexpr int kVMaxThreads = 768;  // 12 waves: up to 168 VGPRs per lane
 
@@ -1367,19 +1368,22 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     // FULL: D == Dp == 64 CH, every chunk of every lane is real (no padding
     // chunk, no half chunk), so no per-chunk lane conditions are needed.
     constexpr bool keep_sum = KEEP;
-    constexpr int KL = kVKL, NP = 8 * CH, SWL = KL * NP + 4;
-    extern __shared__ uint32_t xs[];               // [FL, FR][nt][SWL]
+    constexpr int KL = kVKL, NP = 8 * CH;
+    extern __shared__ uint32_t xs[];               // FL slots, then FR slots
     const int ri = find_rig(bs, nrig, blockIdx.x);
     const RigDesc& g = rigs[ri];
     const int Wp = g.Wp, Hp = g.Hp, D = g.D, Dp = g.Dp;
     const int t0 = seg * kVGhost;
     if (t0 >= Hp) return;                          // the rig has no such segment: whole block
     const int nrows = min(kVGhost, Hp - t0);
-    const int nt = ncol + 2 * kVGhost;
+    (void)ncol;                                    // kVCols (compile-time LDS strides)
+    constexpr int nt = kVCols + 2 * kVGhost;
     const int tid = threadIdx.x, c = tid / KL;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = (int)(blockDim.x >> 6);
-    const int x = (blockIdx.x - bs[ri]) * ncol - kVGhost + c;
+    const int x = (blockIdx.x - bs[ri]) * kVCols - kVGhost + c;
     const bool act = x >= 0 && x < Wp;
+    // every column of the wave inside the image: no border guards to apply
+    const bool wave_act = __builtin_amdgcn_ballot_w64(!act) == 0;
     const bool lghost = wave == 0, rghost = wave == nwaves - 1, own = !lghost && !rghost;
     const bool doFL = !rghost, doFR = !lghost;
     ChainGeo cg;
@@ -1387,8 +1391,12 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     cg.nq = FULL ? KL * CH : Dp >> 4;
     cg.half_tail = FULL ? false : (D & 15) != 0;
     const int boff = cg.lid * CH * 16;
+    // LDS slots of a family, lane-interleaved so that a wave's accesses are
+    // contiguous: pairs 2m, 2m+1 of lane (column cc, lid) in the 8-byte granule
+    // m * GS + cc * KL + lid, then minL' of column cc at word NP * GS + cc
+    constexpr int GS = nt * KL;
     uint32_t* XL = xs;
-    uint32_t* XR = xs + nt * SWL;
+    uint32_t* XR = xs + NP * GS + nt;
     constexpr int famL = DIR == 0 ? 2 : 3, famR = DIR == 0 ? 3 : 2;   // family ids of FL, FR
     const int SW = Dp / 2 + 4;
     uint32_t* const Sg = (uint32_t*)g.part[3];
@@ -1422,27 +1430,27 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         });
         if (cg.lid == 0) G(v)[Dp / 2] = st.mp;
     };
-    // LDS slots
     auto slot_write = [&](uint32_t* X, const LaneState<CH>& st) __attribute__((always_inline)) {
-        uint32_t* w = X + c * SWL + cg.lid * NP;
-        static_for<0, 2 * CH>([&](auto kc) {
+        uint2* w = (uint2*)X + c * KL + cg.lid;
+        static_for<0, NP / 2>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            *(uint4*)(w + 4 * k) = make_uint4(st.L[4 * k], st.L[4 * k + 1], st.L[4 * k + 2], st.L[4 * k + 3]);
+            w[k * GS] = make_uint2(st.L[2 * k], st.L[2 * k + 1]);
         });
-        if (cg.lid == 0) X[c * SWL + KL * NP] = st.mp;
+        if (cg.lid == 0) X[NP * GS + c] = st.mp;
     };
     auto slot_read = [&](const uint32_t* X, int cc, LaneState<CH>& st) __attribute__((always_inline)) {
         if (cc < 0 || cc >= nt) {   // beyond the computed columns: a border state
             vs_guard_all<CH>(st);
             return;
         }
-        const uint32_t* w = X + cc * SWL + cg.lid * NP;
-        static_for<0, 2 * CH>([&](auto kc) {
+        const uint2* w = (const uint2*)X + cc * KL + cg.lid;
+        static_for<0, NP / 2>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            const uint4 v = *(const uint4*)(w + 4 * k);
-            st.L[4 * k] = v.x; st.L[4 * k + 1] = v.y; st.L[4 * k + 2] = v.z; st.L[4 * k + 3] = v.w;
+            const uint2 v = w[k * GS];
+            st.L[2 * k] = v.x;
+            st.L[2 * k + 1] = v.y;
         });
-        st.mp = X[cc * SWL + KL * NP];
+        st.mp = X[NP * GS + cc];
     };
     auto row_of = [&](int t) { return DIR == 0 ? t : Hp - 1 - t; };
     auto load_m = [&](int t, uint4 (&b)[CH]) __attribute__((always_inline)) {
@@ -1549,29 +1557,36 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         // (the steps' L - M outputs are not kept: their sum is formed after the
         // three steps from the new states, sum_f (L_f - M) = sum_f L_f - 3 M,
         // which keeps 24 registers free while the states are stepped)
+        // Two barriers per row: FL slots are read (old) before the first and
+        // written after it; FR slots read between the two and written after
+        // the second.  The next row's FL reads follow this row's second
+        // barrier, its FR reads its own first one, which orders them after
+        // this row's writes.
         uint32_t dead[CH][8];
+        if (doFL) slot_read(XL, c - 1, sx);
+        __syncthreads();
+        if (doFL) {
+            lane_step_pk<CH, KL>(sx, cg, mc, dead);
+            if (!wave_act) lane_guard<CH>(sx, act);
+            slot_write(XL, sx);
+            if (own && edge_row && keep_sum && g.sum && act) edge_out(famL, y, sx);
+        }
+#ifndef SISTER_VS_INTERLEAVE
+        __builtin_amdgcn_sched_barrier(0);
+#endif
         if (own) {
             lane_step_pk<CH, KL>(sc, cg, mc, dead);
             if (edge_row && keep_sum && g.sum && act) edge_out(1, y, sc);
         }
         __builtin_amdgcn_sched_barrier(0);
-        if (doFL) slot_read(XL, c - 1, sx);
-        __syncthreads();
-        if (doFL) {
-            lane_step_pk<CH, KL>(sx, cg, mc, dead);
-            lane_guard<CH>(sx, act);
-            slot_write(XL, sx);
-            if (own && edge_row && keep_sum && g.sum && act) edge_out(famL, y, sx);
-        }
         if (doFR) slot_read(XR, c + 1, sx);
         __syncthreads();
         if (doFR) {
             lane_step_pk<CH, KL>(sx, cg, mc, dead);
-            lane_guard<CH>(sx, act);
+            if (!wave_act) lane_guard<CH>(sx, act);
             slot_write(XR, sx);
             if (own && edge_row && keep_sum && g.sum && act) edge_out(famR, y, sx);
         }
-        __syncthreads();
         if (!own || !act) continue;
         if (edge_row) {
             if (DIR == 1 && keep_sum && g.sum) edge_sum(y);   // y == 0
@@ -1582,12 +1597,15 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         // hoisting every chunk's LDS reads at once)
         auto chunk_sum = [&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            const uint32_t* wl = XL + c * SWL + cg.lid * NP + 8 * q;
-            const uint32_t* wr = XR + c * SWL + cg.lid * NP + 8 * q;
-            const uint4 l0 = *(const uint4*)wl, l1 = *(const uint4*)(wl + 4);
-            const uint4 r0 = *(const uint4*)wr, r1 = *(const uint4*)(wr + 4);
-            const uint32_t lv[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
-            const uint32_t rv[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+            const uint2* wl = (const uint2*)XL + c * KL + cg.lid + 4 * q * GS;
+            const uint2* wr = (const uint2*)XR + c * KL + cg.lid + 4 * q * GS;
+            uint32_t lv[8], rv[8];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint2 a = wl[k * GS], b = wr[k * GS];
+                lv[2 * k] = a.x; lv[2 * k + 1] = a.y;
+                rv[2 * k] = b.x; rv[2 * k + 1] = b.y;
+            }
             uint32_t mq[8];
             unpack16(mc[q], mq);
 #pragma unroll
@@ -1909,7 +1927,8 @@ static int vs_ch(int D) { return ((D + 15) / 16 + kVKL - 1) / kVKL; }
 
 static size_t vs_lds(int D, int vcols)
 {
-    return (size_t)2 * (vcols + 2 * kVGhost) * (32 * vs_ch(D) + 4) * sizeof(uint32_t);
+    const size_t nt = vcols + 2 * kVGhost;
+    return 2 * (8 * vs_ch(D) * nt * kVKL + nt) * sizeof(uint32_t);
 }
 
 int vsweep_pick_cols(long cols, int D, int choice)
@@ -1924,7 +1943,7 @@ int vsweep_pick_cols(long cols, int D, int choice)
     // up to 192 disparities (3 chunks per lane): beyond, the state spills
     if (forced == 0 || vs_ch(D) > 3) return 0;
     // 144 owned columns: 176 computed, 11 waves; 16 blocks per 2304-wide rig
-    const int vcols = 144;
+    const int vcols = kVCols;
     if (vs_lds(D, vcols) > g_lds_per_block) return 0;
     // enough blocks to fill the chip (few rigs keep the per-chain families,
     // which spread a single rig over every CU)
