@@ -7,7 +7,8 @@ namespace sister {
 
 enum Stage {
     kStageCensus = 0,
-    kStageWta,
+    kStageWta,          // all WTA blocks, or (row form) the rows of frames 0/2
+    kStageWtaV,         // row form: the rows of frames 1/3
     kStageMedianPrep,
     kStageMask,
     kNumStages
@@ -43,6 +44,8 @@ struct LaunchSet {
     uint64_t* const* scratch;       // per rig: 8 median order-statistic maps
     int maxD;
     int maxCols;                    // max frame width over the rigs (median LDS)
+    int maxWp, maxHp;               // max padded width / height over the rigs
+    bool wta_rows;                  // WTA by frame rows from shared costs (k_wta_row)
     int ngroups;
     DGroup groups[kMaxDGroups];
 };
@@ -61,7 +64,8 @@ struct KernelTimer {
     virtual ~KernelTimer() {}
 };
 
-void stage_blocks(const RigDesc& g, int maxD, int out[kNumStages]);
+bool wta_rows_ok(int maxCols, int maxD);
+void stage_blocks(const RigDesc& g, int maxD, bool wta_rows, int out[kNumStages]);
 int sgm_pick_kl(long cols_chains, int fam, int D);
 int sgm_pick_ch(int D, int KL);
 int sgm_family_blocks(const RigDesc& g, int KL, int fam);

@@ -50,6 +50,16 @@ __device__ __forceinline__ int find_rig(const int* __restrict__ bs, int n, int b
     return lo;
 }
 
+// Popcounts accumulate in place: v_bcnt_u32_b32 adds its second operand, one
+// instruction per 32-bit half (written out, since the compiler reassociates
+// pop + pop + acc into two plain popcounts and a three-way add).
+__device__ __forceinline__ unsigned bcnt_add(uint32_t x, unsigned acc)
+{
+    unsigned r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+
 // padded original (y, x) of frame pixel (r, c)
 __device__ __forceinline__ void frame_to_orig(const RigDesc& g, int F, int r, int c, int& y, int& x)
 {
@@ -242,6 +252,123 @@ __global__ __launch_bounds__(256) void k_wta(const RigDesc* __restrict__ rigs, i
     }
     g.wta[2 * F][(size_t)r * C + c] = (uint16_t)(bestL & 511u);
     g.wta[2 * F + 1][(size_t)r * C + c] = (uint16_t)(bestR & 511u);
+}
+
+// WTALeft and WTARight of one frame row from shared costs (same results as
+// k_wta).  The right map's candidates for pixel p are cost(p + k, k), the
+// left map's for pixel c are cost(c, d): one cost serves both.  One block per
+// frame row, the row's codes in LDS; a wave takes 256-pixel segments, lane l
+// the 4 pixels c = x0 + 4l + i, and steps d = 0..D-1:
+//   w[i]  = CN[c_i - d]        partner codes: shift by one pixel per step,
+//                              one new LDS read per step for 4 costs
+//   key   = (pop(CC[c_i] ^ w[i]) << 9) | d        (lowest index wins ties)
+//   L[i]  = min(L[i], key)                         left map of pixel c_i
+//   R[i]  = min(R[i], key)                         right map of pixel c_i - d
+// R[i] belongs to pixel x0 + 4l + i - d, so the right accumulators move one
+// pixel per step: R[3] of lane l becomes R[0] of lane l + 1 (DPP wave_shr:1,
+// a fresh accumulator entering lane 0).  Accumulators leaving lane 63, and
+// all of them after the last step, are partial minima (their pixel's other
+// candidates lie in neighbouring segments): they meet by LDS atomic min, once
+// per pixel and segment.  About 3.6 VALU per evaluation against 6.
+// ---------------------------------------------------------------------------
+constexpr int kWtaRowMaxWaves = 16;
+
+size_t wta_row_lds(int C, int D)
+{
+    return ((size_t)C + (size_t)C + D - 1 + 256) * sizeof(uint64_t) + (size_t)C * sizeof(uint32_t);
+}
+
+template <bool EDGE>
+__device__ __forceinline__ void wta_row_segment(const uint64_t* ccS, const uint64_t* cnS, uint32_t* RP,
+                                                uint16_t* outL, int x0, int C, int D, int lane)
+{
+    const int cb = x0 + 4 * lane;
+    uint64_t cci[4], w[4];
+    uint32_t bL[4], bR[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        cci[i] = (!EDGE || cb + i < C) ? ccS[cb + i] : 0;
+        w[i] = cnS[cb + i + D - 1];
+        bL[i] = 0xFFFFFFFFu;
+        bR[i] = 0xFFFFFFFFu;
+    }
+    auto step = [&](int d) __attribute__((always_inline)) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint64_t x = cci[i] ^ w[i];
+            const uint32_t pop = bcnt_add((uint32_t)(x >> 32), bcnt_add((uint32_t)x, 0u));
+            const uint32_t key = (pop << 9) | (uint32_t)d;
+            bL[i] = min(bL[i], (EDGE && d > cb + i) ? 0xFFFFFFFFu : key);
+            bR[i] = min(bR[i], (EDGE && cb + i >= C) ? 0xFFFFFFFFu : key);
+        }
+    };
+    auto shift = [&](int d) __attribute__((always_inline)) {
+        // to step d + 1: R accumulators move up one pixel (one lane per 4)
+        const uint32_t out = bR[3];
+        const uint32_t in = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)out, 0x138, 0xF, 0xF, false);
+        const int pl = x0 + 255 - d;          // pixel of lane 63's leaving accumulator
+        if (lane == 63 && pl < C) atomicMin(&RP[pl], out);
+        bR[3] = bR[2]; bR[2] = bR[1]; bR[1] = bR[0]; bR[0] = in;
+        w[3] = w[2]; w[2] = w[1]; w[1] = w[0];
+        w[0] = cnS[cb + D - 2 - d];           // CN[cb - (d + 1)]
+    };
+    for (int d = 0; d < D; d += 4) {
+        step(d);     shift(d);
+        step(d + 1); shift(d + 1);
+        step(d + 2); shift(d + 2);
+        step(d + 3);
+        if (d + 4 < D) shift(d + 3);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (!EDGE || cb + i < C) G(outL)[cb + i] = (uint16_t)(bL[i] & 511u);
+        const int pp = cb + i - (D - 1);      // pixel of R[i] after the last step
+        if (pp >= 0 && pp < C) atomicMin(&RP[pp], bR[i]);
+    }
+}
+
+template <bool V>
+__global__ __launch_bounds__(64 * kWtaRowMaxWaves) void k_wta_row(const RigDesc* __restrict__ rigs, int nrig,
+                                                                 const int* __restrict__ bs)
+{
+    extern __shared__ uint64_t lds64[];
+    const int ri = find_rig(bs, nrig, blockIdx.x);
+    const RigDesc& g = rigs[ri];
+    const int lb = blockIdx.x - bs[ri];
+    const int D = g.D;
+    // V: frames 1 and 3 (W' rows of H' pixels), else frames 0 and 2 (H' rows of W')
+    const int nrow = V ? g.Wp : g.Hp, C = V ? g.Hp : g.Wp;
+    const int F = (V ? 1 : 0) + (lb >= nrow ? 2 : 0);
+    const int r = lb >= nrow ? lb - nrow : lb;
+    const uint64_t* cc = g.code[2 * F] + (size_t)r * C;
+    const uint64_t* cn = g.code[2 * F + 1] + (size_t)r * C;
+    uint64_t* ccS = lds64;                       // ccS[c] = CC[c]
+    uint64_t* cnS = lds64 + C;                   // cnS[i] = CN[i - (D - 1)], zero outside
+    uint32_t* RP = (uint32_t*)(cnS + C + D - 1 + 256);   // right-map partial keys
+    const int ncn = C + D - 1 + 256;
+    for (int i = threadIdx.x; i < ncn; i += blockDim.x) {
+        const int a = i - (D - 1);
+        cnS[i] = (a >= 0 && a < C) ? G(cn)[a] : 0;
+    }
+    for (int i = threadIdx.x; i < C; i += blockDim.x) {
+        ccS[i] = G(cc)[i];
+        RP[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int nseg = (C + 255) >> 8;
+    uint16_t* const outL = g.wta[2 * F] + (size_t)r * C;
+    for (int sg = wave; sg < nseg; sg += nwaves) {
+        const int x0 = sg * 256;
+        // segments at the row's ends mask the candidates that do not exist
+        // (fewer than D at the start, pixels past the row at the end)
+        const bool lfix = x0 < D - 1, rfix = x0 + 256 > C;
+        if (lfix || rfix) wta_row_segment<true>(ccS, cnS, RP, outL, x0, C, D, lane);
+        else wta_row_segment<false>(ccS, cnS, RP, outL, x0, C, D, lane);
+    }
+    __syncthreads();
+    uint16_t* const outR = g.wta[2 * F + 1] + (size_t)r * C;
+    for (int p = threadIdx.x; p < C; p += blockDim.x) G(outR)[p] = (uint16_t)(RP[p] & 511u);
 }
 
 // ---------------------------------------------------------------------------
@@ -502,16 +629,6 @@ constexpr int kWinPad = 8;  // window entries before each partner window
 // acc[j] += pop(cc ^ window[base - d]) for lane q's disparities d = q + 64 j.
 // In the last group, lanes at or past Dp read as lane qlast = Dp - 1 - 64 (J-1)
 // (their sums are not stored) and lanes at or past D are zeroed by the caller.
-// Popcounts accumulate in place: v_bcnt_u32_b32 adds its second operand, one
-// instruction per 32-bit half (written out, since the compiler reassociates
-// pop + pop + acc into two plain popcounts and a three-way add).
-__device__ __forceinline__ unsigned bcnt_add(uint32_t x, unsigned acc)
-{
-    unsigned r;
-    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
-    return r;
-}
-
 template <int J>
 __device__ __forceinline__ void msum_add_frame(const uint64_t* L, uint64_t cc, int base, int q, int qlast,
                                                unsigned (&acc)[J])
@@ -1802,9 +1919,19 @@ void launch_census(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
 
 void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
 {
-    size_t lds = (size_t)2 * (256 + ls.maxD - 1) * sizeof(uint64_t);
     TB(kKWta);
-    hipLaunchKernelGGL(k_wta, dim3(ls.blocks[kStageWta]), dim3(256), lds, st, ls.rigs, ls.nrig, ls.bs[kStageWta]);
+    if (ls.wta_rows) {
+        // frames 0/2 (rows of W' pixels), then frames 1/3 (rows of H')
+        const int waves02 = std::min(kWtaRowMaxWaves, (ls.maxWp + 255) / 256);
+        const int waves13 = std::min(kWtaRowMaxWaves, (ls.maxHp + 255) / 256);
+        hipLaunchKernelGGL(k_wta_row<false>, dim3(ls.blocks[kStageWta]), dim3(64 * waves02),
+                           wta_row_lds(ls.maxWp, ls.maxD), st, ls.rigs, ls.nrig, ls.bs[kStageWta]);
+        hipLaunchKernelGGL(k_wta_row<true>, dim3(ls.blocks[kStageWtaV]), dim3(64 * waves13),
+                           wta_row_lds(ls.maxHp, ls.maxD), st, ls.rigs, ls.nrig, ls.bs[kStageWtaV]);
+    } else {
+        size_t lds = (size_t)2 * (256 + ls.maxD - 1) * sizeof(uint64_t);
+        hipLaunchKernelGGL(k_wta, dim3(ls.blocks[kStageWta]), dim3(256), lds, st, ls.rigs, ls.nrig, ls.bs[kStageWta]);
+    }
     TE(kKWta);
 }
 
@@ -1862,6 +1989,12 @@ int kernels_setup(int device)
     (const void*)k_vsweep<1, 0, K, F>, (const void*)k_vsweep<2, 0, K, F>, (const void*)k_vsweep<3, 0, K, F>, \
         (const void*)k_vsweep<4, 0, K, F>, (const void*)k_vsweep<1, 1, K, F>, (const void*)k_vsweep<2, 1, K, F>, \
         (const void*)k_vsweep<3, 1, K, F>, (const void*)k_vsweep<4, 1, K, F>
+    for (const void* f : {(const void*)k_wta_row<false>, (const void*)k_wta_row<true>})
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
+            (void)hipGetLastError();
+            g_lds_per_block = 0;
+            return 0;
+        }
     const void* vfns[24] = {SISTER_VS_FNS(false, false), SISTER_VS_FNS(false, true), SISTER_VS_FNS(true, false)};
 #undef SISTER_VS_FNS
     for (const void* f : vfns)
@@ -2076,12 +2209,29 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
 }
 
 // per-rig block counts of each stage (the host builds the block-start tables)
-void stage_blocks(const RigDesc& g, int maxD, int out[kNumStages])
+bool wta_rows_ok(int maxCols, int maxD)
+{
+    // SISTER_WTA_ROWS=0 keeps the per-segment kernel (comparison runs)
+    static const int env = [] {
+        const char* e = getenv("SISTER_WTA_ROWS");
+        return e ? atoi(e) : -1;
+    }();
+    if (env == 0) return false;
+    return (maxCols + 255) / 256 <= 4 * kWtaRowMaxWaves && wta_row_lds(maxCols, maxD) <= g_lds_per_block;
+}
+
+void stage_blocks(const RigDesc& g, int maxD, bool wta_rows, int out[kNumStages])
 {
     (void)maxD;
     const long npx = (long)g.Wp * g.Hp;
     out[kStageCensus] = 8 * ((g.Wp + kCT - 1) / kCT) * ((g.Hp + kCT - 1) / kCT);
-    out[kStageWta] = 2 * g.Hp * ((g.Wp + 255) / 256) + 2 * g.Wp * ((g.Hp + 255) / 256);
+    if (wta_rows) {
+        out[kStageWta] = 2 * g.Hp;     // rows of frames 0 and 2
+        out[kStageWtaV] = 2 * g.Wp;    // rows of frames 1 and 3
+    } else {
+        out[kStageWta] = 2 * g.Hp * ((g.Wp + 255) / 256) + 2 * g.Wp * ((g.Hp + 255) / 256);
+        out[kStageWtaV] = 0;
+    }
     out[kStageMedianPrep] = (int)(8 * ((npx + 255) / 256));
     out[kStageMask] = (int)(4 * ((npx + 255) / 256));
 }

@@ -334,9 +334,10 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         scr[k] = (uint64_t*)(p.workspaces[k].ptr + l.scratch);
     }
     for (int st = 0; st < kNumStages; st++) p.ls.blocks[st] = 0;
+    const bool wta_rows = wta_rows_ok(maxCols, maxD);
     for (int k = 0; k < n; k++) {
         int nb[kNumStages];
-        stage_blocks(p.host_rigs[k], maxD, nb);
+        stage_blocks(p.host_rigs[k], maxD, wta_rows, nb);
         for (int st = 0; st < kNumStages; st++) {
             bs[st * (n + 1) + k] = p.ls.blocks[st];
             p.ls.blocks[st] += nb[st];
@@ -373,6 +374,13 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         p.ls.bs[st] = (const int*)(p.table.ptr + rig_bytes + scr_bytes) + st * (n + 1);
     p.ls.maxD = maxD;
     p.ls.maxCols = maxCols;
+    p.ls.wta_rows = wta_rows;
+    p.ls.maxWp = 8;
+    p.ls.maxHp = 8;
+    for (const RigDesc& g : p.host_rigs) {
+        p.ls.maxWp = std::max(p.ls.maxWp, g.Wp);
+        p.ls.maxHp = std::max(p.ls.maxHp, g.Hp);
+    }
     p.ls.ngroups = (int)groups.size();
     const int* dg = (const int*)(p.table.ptr + rig_bytes + scr_bytes + bs_bytes);
     for (size_t gi = 0; gi < groups.size(); gi++) {
