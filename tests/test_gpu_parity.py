@@ -108,6 +108,23 @@ def test_stages_bit_exact_fused_vertical(gpu_ctx, oracle_mod, W, H, D, seed):
     assert not bad, bad
 
 
+# the fused sweep's full-lane form (D a multiple of 64, no debug sums: cost
+# rows, rows-family bytes and partial sums in row planes): outputs only
+@pytest.mark.parametrize("W,H,D,seed", [(96, 64, 64, 13), (64, 48, 128, 14), (64, 40, 192, 15), (400, 40, 64, 16)])
+def test_fused_vertical_full_lanes_outputs(gpu_ctx, oracle_mod, W, H, D, seed):
+    print(f"[parity fused full] W={W} H={H} D={D} seed={seed}", flush=True)
+    rig = synth.rig_small(W, H, D, seed)
+    ref = oracle_mod.compute(rig["views"], D, modes=7)
+    gpu_ctx.vsweep(1)
+    try:
+        got = gpu_ctx.compute_rig(rig["views"], D, modes=7)
+    finally:
+        gpu_ctx.vsweep(-1)
+    for k in ("multiview", "horizontal", "vertical", "disp"):
+        assert np.array_equal(got[k], ref[k]), (k, int((got[k] != ref[k]).sum()))
+    assert float(np.abs(got["subpix"] - ref["subpix"]).max()) <= 1e-3
+
+
 def test_plain_640x480_d64(gpu_ctx, oracle_mod):
     print("[parity] 640x480 D=64 seed=1", flush=True)
     rig = synth.rig_plain_640()
