@@ -1026,7 +1026,7 @@ struct ChainGeo {
 // one standard step over the lane's chunks.  m = cost pairs per chunk;
 // returns (L - M) pairs in tp (ACC: adds them to tp).  trunc: 8-bit
 // first-line truncation.
-template <int CH, int KL, bool ACC, class MF>
+template <int CH, int KL, bool ACC, class MF, bool DEFER = false>
 __device__ __forceinline__ void lane_step_f(LaneState<CH>& st, const ChainGeo& cg, MF&& mpair,
                                             uint32_t (&tp)[CH][8], bool trunc);
 
@@ -1037,18 +1037,21 @@ __device__ __forceinline__ void lane_step(LaneState<CH>& st, const ChainGeo& cg,
     lane_step_f<CH, KL, ACC>(st, cg, [&](int q, int i) __attribute__((always_inline)) { return m[q][i]; }, tp, trunc);
 }
 
-// the same step with the cost pairs taken from packed bytes (4 per word, 16 per chunk)
-template <int CH, int KL, bool ACC = false>
+// the same step with the cost pairs taken from packed bytes (4 per word, 16 per chunk).
+// DEFER: the state is kept raw (minL' not subtracted): the new raw values are
+// the normalized ones plus the previous raw minimum, which the caller tracks.
+template <int CH, int KL, bool ACC = false, bool DEFER = false>
 __device__ __forceinline__ void lane_step_pk(LaneState<CH>& st, const ChainGeo& cg, const uint4 (&mb)[CH],
                                              uint32_t (&tp)[CH][8])
 {
-    lane_step_f<CH, KL, ACC>(st, cg, [&](int q, int i) __attribute__((always_inline)) {
+    auto mf = [&](int q, int i) __attribute__((always_inline)) {
         const uint32_t w = i < 2 ? mb[q].x : i < 4 ? mb[q].y : i < 6 ? mb[q].z : mb[q].w;
         return (i & 1) ? bytes23(w) : bytes01(w);
-    }, tp, false);
+    };
+    lane_step_f<CH, KL, ACC, decltype(mf)&, DEFER>(st, cg, mf, tp, false);
 }
 
-template <int CH, int KL, bool ACC, class MF>
+template <int CH, int KL, bool ACC, class MF, bool DEFER>
 __device__ __forceinline__ void lane_step_f(LaneState<CH>& st, const ChainGeo& cg, MF&& mpair,
                                             uint32_t (&tp)[CH][8], bool trunc)
 {
@@ -1073,7 +1076,7 @@ __device__ __forceinline__ void lane_step_f(LaneState<CH>& st, const ChainGeo& c
             const us2 right = U2(__builtin_amdgcn_alignbit(nxt, cur, 16));
             const us2 nb = umin2(left, right) + (us2){(unsigned short)kP1, (unsigned short)kP1};
             const us2 t = umin2(umin2(nb, U2(cur)), p2t);
-            const us2 d = t - U2(mpp);
+            const us2 d = DEFER ? t : t - U2(mpp);
             us2 l = d + U2(mpair(q, i));
             if (trunc) l = l & (us2){0xFF, 0xFF};
             const bool pad = !valid || (ht && i >= 4);
@@ -1485,6 +1488,12 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     // FULL: D == Dp == 64 CH, every chunk of every lane is real (no padding
     // chunk, no half chunk), so no per-chunk lane conditions are needed.
     constexpr bool keep_sum = KEEP;
+    // deferred normalisation: within a segment the states stay raw (the step
+    // does not subtract minL'); the row's output subtracts the three
+    // families' pre-step minima once, and the segment-end store normalises.
+    // Raw values grow by at most 255 per row, so 16 rows stay far below the
+    // guard 0x3FFF.  The debug variant keeps exact states for its edge rows.
+    constexpr bool DEFER = !KEEP;
     constexpr int KL = kVKL, NP = 8 * CH;
     extern __shared__ uint32_t xs[];               // FL slots, then FR slots
     const int ri = find_rig(bs, nrig, blockIdx.x);
@@ -1554,11 +1563,14 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
             constexpr int q = decltype(qc)::value;
             const int qg = cg.lid * CH + q;
             if (FULL || qg < cg.nq) {
-                ST16(v + qg * 8, make_uint4(st.L[8 * q + 0], st.L[8 * q + 1], st.L[8 * q + 2], st.L[8 * q + 3]));
-                ST16(v + qg * 8 + 4, make_uint4(st.L[8 * q + 4], st.L[8 * q + 5], st.L[8 * q + 6], st.L[8 * q + 7]));
+                // DEFER: stored minus minL' (guard pairs stay above 0x3FFF - 5000)
+                const us2 n = DEFER ? U2(dup16(st.mp)) : (us2){0, 0};
+                auto w = [&](int k) __attribute__((always_inline)) { return W2(U2(st.L[8 * q + k]) - n); };
+                ST16(v + qg * 8, make_uint4(w(0), w(1), w(2), w(3)));
+                ST16(v + qg * 8 + 4, make_uint4(w(4), w(5), w(6), w(7)));
             }
         });
-        if (cg.lid == 0) G(v)[Dp / 2] = st.mp;
+        if (cg.lid == 0) G(v)[Dp / 2] = DEFER ? 0u : st.mp;
     };
     auto slot_write = [&](uint32_t* X, const LaneState<CH>& st) __attribute__((always_inline)) {
         uint2* w = (uint2*)X + c * KL + cg.lid;
@@ -1695,8 +1707,10 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         uint32_t dead[CH][8];
         if (doFL) slot_read(XL, c - 1, sx);
         __syncthreads();
+        uint32_t off = 0;   // DEFER: sum of the three pre-step minima
         if (doFL) {
-            lane_step_pk<CH, KL>(sx, cg, mc, dead);
+            off = sx.mp;
+            lane_step_pk<CH, KL, false, DEFER>(sx, cg, mc, dead);
             if (!wave_act) lane_guard<CH>(sx, act);
             slot_write(XL, sx);
             if (own && edge_row && keep_sum && g.sum && act) edge_out(famL, y, sx);
@@ -1705,14 +1719,16 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         __builtin_amdgcn_sched_barrier(0);
 #endif
         if (own) {
-            lane_step_pk<CH, KL>(sc, cg, mc, dead);
+            off += sc.mp;
+            lane_step_pk<CH, KL, false, DEFER>(sc, cg, mc, dead);
             if (edge_row && keep_sum && g.sum && act) edge_out(1, y, sc);
         }
         __builtin_amdgcn_sched_barrier(0);
         if (doFR) slot_read(XR, c + 1, sx);
         __syncthreads();
         if (doFR) {
-            lane_step_pk<CH, KL>(sx, cg, mc, dead);
+            off += sx.mp;
+            lane_step_pk<CH, KL, false, DEFER>(sx, cg, mc, dead);
             if (!wave_act) lane_guard<CH>(sx, act);
             slot_write(XR, sx);
             if (own && edge_row && keep_sum && g.sum && act) edge_out(famR, y, sx);
@@ -1722,7 +1738,7 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
             if (DIR == 1 && keep_sum && g.sum) edge_sum(y);   // y == 0
             continue;
         }
-        // acc = (L_cols + L_FL + L_FR) - 3 M from the new states of this column,
+        // acc = (L_cols + L_FL + L_FR) - 3 M (- the pre-step minima) from the new states of this column,
         // one chunk at a time (scheduling barriers keep the compiler from
         // hoisting every chunk's LDS reads at once)
         auto chunk_sum = [&](auto qc) __attribute__((always_inline)) {
@@ -1738,9 +1754,10 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
             }
             uint32_t mq[8];
             unpack16(mc[q], mq);
+            const us2 o2 = DEFER ? U2(dup16(off)) : (us2){0, 0};
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const us2 m3 = U2(mq[k]) * (us2){3, 3};
+                const us2 m3 = U2(mq[k]) * (us2){3, 3} + o2;
                 acc[q][k] = W2(U2(sc.L[8 * q + k]) + U2(lv[k]) + U2(rv[k]) - m3);
             }
         };
