@@ -121,20 +121,29 @@ __device__ __forceinline__ unsigned wave_min(unsigned v)
 }
 
 // ---------------------------------------------------------------------------
-// K1 census (census.cpp:38-51), one 32x32 tile of frame pixels per block (4
-// pixels per thread, so that staging the tile's window costs less per pixel).
+// K1 census (census.cpp:38-51), one 32x32 tile of frame pixels per block.
 // Test bit for offset (y,x), y=-3..3 outer, x=-4..4 inner, is
 // I(r+y, c+x) > I(r-y, c-x), shifted in MSB-first; bit 63 carries bit 0 of
 // the previous domain pixel in frame raster order (the accumulator is never
 // reset, census.cpp:42), i.e. the (3,4) test of that pixel.
+// The tile (frame rows r0-3 .., columns c0-8 ..) is staged in LDS as bytes,
+// rows 40 dwords apart (40 = 8 mod 32 banks: the 32 lanes of a half-wave,
+// 4 rows x 8 runs, read 32 different banks).  A thread takes a run of 4
+// frame pixels of one row and reads each of its 7 window rows once as 3 or 4
+// aligned dwords; one alignbyte gathers a test's 4 left-hand bytes (one per
+// pixel), another the 4 right-hand bytes, and each pixel's two bits come from
+// two byte-select subtractions.
 // ---------------------------------------------------------------------------
 constexpr int kCT = 32;   // tile side (frame pixels)
 
 __global__ __launch_bounds__(256) void k_census(const RigDesc* __restrict__ rigs, int nrig,
                                                 const int* __restrict__ bs)
 {
-    constexpr int TR = kCT + 6, TC = kCT + 9;   // rows r0-3 .., cols c0-5 .. (c0-5: the carry test)
-    __shared__ uint8_t tile[TR][TC];
+    constexpr int TR = kCT + 6;       // frame rows r0-3 .. r0+kCT+2
+    constexpr int TB = kCT + 12;      // bytes used per row: columns c0-8 .. c0+kCT+3
+    constexpr int TW = 40;            // dwords per tile row
+    __shared__ uint32_t tile[TR * TW];
+    uint8_t* const tb = (uint8_t*)tile;
     const int ri = find_rig(bs, nrig, blockIdx.x);
     const RigDesc& g = rigs[ri];
     const int lb = blockIdx.x - bs[ri];
@@ -149,50 +158,65 @@ __global__ __launch_bounds__(256) void k_census(const RigDesc* __restrict__ rigs
     // frames 90/270 are transposes: step along frame rows there, so that
     // consecutive lanes read along an image row
     const bool odd = F & 1;
-    for (int i = threadIdx.x; i < TR * TC; i += 256) {
-        const int rr = odd ? i % TR : i / TC, cc = odd ? i / TR : i % TC;
-        const int r = r0 - 3 + rr, c = c0 - 5 + cc;
-        tile[rr][cc] = (r >= 0 && r < R && c >= 0 && c < C) ? (uint8_t)frame_px(g, v, F, r, c) : 0;
+    for (int i = threadIdx.x; i < TR * TB; i += 256) {
+        const int rr = odd ? i % TR : i / TB, cc = odd ? i / TR : i % TB;
+        const int r = r0 - 3 + rr, c = c0 - 8 + cc;
+        tb[rr * TW * 4 + cc] = (r >= 0 && r < R && c >= 0 && c < C) ? (uint8_t)frame_px(g, v, F, r, c) : 0;
     }
     __syncthreads();
-    const int c = c0 + (threadIdx.x & 31);
-    if (c >= C) return;
-    uint64_t* out = g.code[unit];
+    // run of pixels (r, cb .. cb+3); C is a multiple of 4, so a run is all in or all out
+    const int run = threadIdx.x & 7, rl = threadIdx.x >> 3;
+    const int r = r0 + rl, cb = c0 + 4 * run;
+    if (r >= R || cb >= C) return;
+    // w[i][k]: dword run + k of tile row rl + i (frame row r - 3 + i); byte b of
+    // the run's window is frame column cb - 8 + b
+    uint32_t w[7][4];
 #pragma unroll
-    for (int k = 0; k < kCT / 8; k++) {
-        const int r = r0 + (threadIdx.x >> 5) + 8 * k;
-        if (r >= R) break;
-        uint64_t code = 0;
+    for (int i = 0; i < 7; i++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) w[i][k] = tile[(rl + i) * TW + run + k];
+    // bytes s .. s+3 of row i: byte p is pixel p's entry at window byte s + p
+    auto quad = [&](int i, int sb) __attribute__((always_inline)) {
+        return (sb & 3) == 0 ? w[i][sb >> 2] : __builtin_amdgcn_alignbyte(w[i][(sb >> 2) + 1], w[i][sb >> 2], sb & 3);
+    };
+    // test u (offset (u/9 - 3, u%9 - 4)) and its mirror 62 - u compare the same
+    // two pixels in opposite order: bit 62 - u = (a > b) is the sign of b - a,
+    // bit u = (b > a) the sign of a - b.  hi collects bits 62..32 MSB-first;
+    // lf collects bits 0..30 in the same order and is bit-reversed at the end.
+    uint32_t hi[4] = {0, 0, 0, 0}, lf[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < 31; u++) {
+        const int y = u / 9 - 3, x = u % 9 - 4;
+        const uint32_t qa = quad(3 + y, 8 + x), qb = quad(3 - y, 8 - x);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const uint32_t a = (qa >> (8 * p)) & 0xFFu, b = (qb >> (8 * p)) & 0xFFu;
+            hi[p] = __builtin_amdgcn_alignbit(hi[p], b - a, 31);
+            lf[p] = __builtin_amdgcn_alignbit(lf[p], a - b, 31);
+        }
+    }
+    // carry of pixel c: the (3,4) test of pixel c-1, I(r+3, c+3) > I(r-3, c-5)
+    const uint32_t ca = quad(6, 11 + 0), cbq = quad(0, 3);
+    uint64_t code[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int c = cb + p;
+        code[p] = 0;
         if (r >= 3 && r < R - 3 && c >= 4 && c < C - 4) {
-            const int lr = r - r0 + 3, lc = c - c0 + 5;
-            // test t (t = 0..62, offset (t/9 - 3, t%9 - 4)) lands in bit 62 - t.
-            // Tests t and 62 - t compare the same two pixels in opposite order,
-            // so each pair gives both bits: a > b is the sign of b - a, and the
-            // mirror test's bit the sign of a - b.  Tests 0..30 fill bits 62..32
-            // (hi), the center test 31 is always 0 (bit 31), tests 32..62 fill
-            // bits 30..0 (lo); each bit is shifted in with one alignbit.
-            uint32_t hi = 0, lo = 0;
-            int da[31];
-#pragma unroll
-            for (int u = 0; u < 31; u++) {
-                const int y = u / 9 - 3, x = u % 9 - 4;
-                da[u] = (int)tile[lr + y][lc + x] - (int)tile[lr - y][lc - x];
-                hi = __builtin_amdgcn_alignbit(hi, (uint32_t)(-da[u]), 31);
-            }
-#pragma unroll
-            for (int u = 30; u >= 0; u--) lo = __builtin_amdgcn_alignbit(lo, (uint32_t)da[u], 31);
-            code = ((uint64_t)hi << 32) | lo;
+            const uint32_t lo = __builtin_bitreverse32(lf[p]) >> 1;
             unsigned carry;
             if (c > 4)
-                carry = tile[lr + 3][lc + 3] > tile[lr - 3][lc - 5];          // previous pixel (r, c-1)
+                carry = ((ca >> (8 * p)) & 0xFFu) > ((cbq >> (8 * p)) & 0xFFu);
             else if (r > 3)
                 carry = frame_px(g, v, F, r + 2, C - 1) > frame_px(g, v, F, r - 4, C - 9);  // (r-1, C-5)
             else
                 carry = 0;
-            code |= (uint64_t)carry << 63;
+            code[p] = ((uint64_t)(hi[p] | (carry << 31)) << 32) | lo;
         }
-        G(out)[(size_t)r * C + c] = code;
     }
+    uint64_t* out = g.code[unit] + (size_t)r * C + cb;   // 32-byte aligned: cb % 4 == 0
+    ST16(out, make_uint4((uint32_t)code[0], (uint32_t)(code[0] >> 32), (uint32_t)code[1], (uint32_t)(code[1] >> 32)));
+    ST16(out + 2, make_uint4((uint32_t)code[2], (uint32_t)(code[2] >> 32), (uint32_t)code[3], (uint32_t)(code[3] >> 32)));
 }
 
 // ---------------------------------------------------------------------------
