@@ -331,18 +331,25 @@ __device__ __forceinline__ void wta_row_segment(const uint64_t* ccS, const uint6
         const uint32_t out = bR[3];
         const uint32_t in = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)out, 0x138, 0xF, 0xF, false);
         const int pl = x0 + 255 - d;          // pixel of lane 63's leaving accumulator
-        if (lane == 63 && pl < C) atomicMin(&RP[pl], out);
+        if (lane == 63 && (!EDGE || pl < C)) atomicMin(&RP[pl], out);   // interior: x0 + 256 <= C
         bR[3] = bR[2]; bR[2] = bR[1]; bR[1] = bR[0]; bR[0] = in;
         w[3] = w[2]; w[2] = w[1]; w[1] = w[0];
         w[0] = cnS[cb + D - 2 - d];           // CN[cb - (d + 1)]
     };
-    for (int d = 0; d < D; d += 4) {
+    // four steps per iteration (the rotation of w and R repeats every four
+    // shifts, so the registers rename with no moves); D % 8 == 0, and the
+    // last quad, which has no final shift, is peeled
+    int d = 0;
+    for (; d + 4 < D; d += 4) {
         step(d);     shift(d);
         step(d + 1); shift(d + 1);
         step(d + 2); shift(d + 2);
-        step(d + 3);
-        if (d + 4 < D) shift(d + 3);
+        step(d + 3); shift(d + 3);
     }
+    step(d);     shift(d);
+    step(d + 1); shift(d + 1);
+    step(d + 2); shift(d + 2);
+    step(d + 3);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         if (!EDGE || cb + i < C) G(outL)[cb + i] = (uint16_t)(bL[i] & 511u);
