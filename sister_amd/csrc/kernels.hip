@@ -1480,6 +1480,9 @@ constexpr int kVGhost = 16;        // ghost columns per side = rows per launch
 constexpr int kVCols = 144;        // owned columns per block (16 blocks per 2304-wide rig)
 constexpr int kVKL = 4;            // lanes per column
 constexpr int kVMaxThreads = 768;  // 12 waves: up to 168 VGPRs per lane
+// 8-byte granules per pair slot of the family state slots: nt computed
+// columns plus the two border columns, KL lanes each, rounded up to 64
+__host__ __device__ constexpr int vs_gs(int nt) { return ((nt + 2) * kVKL + 63) / 64 * 64; }
 
 template <int CH>
 __device__ __forceinline__ void vs_guard_all(LaneState<CH>& st)
@@ -1541,8 +1544,10 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     const bool act = x >= 0 && x < Wp;
     // every column of the wave inside the image: no border guards to apply
     const bool wave_act = __builtin_amdgcn_ballot_w64(!act) == 0;
-    const bool lghost = wave == 0, rghost = wave == nwaves - 1, own = !lghost && !rghost;
-    const bool doFL = !rghost, doFR = !lghost;
+    // ghost waves step all three families too (their outward-flowing chains
+    // and columns are never read by an owned column), so no wave branches
+    // around a step; the busiest SIMD still carries three owned waves
+    const bool own = wave != 0 && wave != nwaves - 1;
     ChainGeo cg;
     cg.lid = tid % KL;
     cg.nq = FULL ? KL * CH : Dp >> 4;
@@ -1550,10 +1555,14 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     const int boff = cg.lid * CH * 16;
     // LDS slots of a family, lane-interleaved so that a wave's accesses are
     // contiguous: pairs 2m, 2m+1 of lane (column cc, lid) in the 8-byte granule
-    // m * GS + cc * KL + lid, then minL' of column cc at word NP * GS + cc
-    constexpr int GS = nt * KL;
+    // m * GS + (cc + 1) * KL + lid, then minL' of column cc at word
+    // NP * GS + cc + 1.  Columns -1 and nt hold a border state (guards,
+    // minL' = 0), written once and never overwritten, so reads need no test.
+    // GS is a multiple of 64 granules, so that the compiler pairs the slot
+    // accesses (ds_read2st64 / ds_write2st64).
+    constexpr int GS = vs_gs(nt);
     uint32_t* XL = xs;
-    uint32_t* XR = xs + NP * GS + nt;
+    uint32_t* XR = xs + NP * GS + nt + 2;
     constexpr int famL = DIR == 0 ? 2 : 3, famR = DIR == 0 ? 3 : 2;   // family ids of FL, FR
     const int SW = Dp / 2 + 4;
     uint32_t* const Sg = (uint32_t*)g.part[3];
@@ -1604,26 +1613,22 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         if (cg.lid == 0) G(v)[Dp / 2] = DEFER ? 0u : st.mp;
     };
     auto slot_write = [&](uint32_t* X, const LaneState<CH>& st) __attribute__((always_inline)) {
-        uint2* w = (uint2*)X + c * KL + cg.lid;
+        uint2* w = (uint2*)X + (c + 1) * KL + cg.lid;
         static_for<0, NP / 2>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             w[k * GS] = make_uint2(st.L[2 * k], st.L[2 * k + 1]);
         });
-        if (cg.lid == 0) X[NP * GS + c] = st.mp;
+        if (cg.lid == 0) X[NP * GS + c + 1] = st.mp;
     };
     auto slot_read = [&](const uint32_t* X, int cc, LaneState<CH>& st) __attribute__((always_inline)) {
-        if (cc < 0 || cc >= nt) {   // beyond the computed columns: a border state
-            vs_guard_all<CH>(st);
-            return;
-        }
-        const uint2* w = (const uint2*)X + cc * KL + cg.lid;
+        const uint2* w = (const uint2*)X + (cc + 1) * KL + cg.lid;   // -1 <= cc <= nt
         static_for<0, NP / 2>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             const uint2 v = w[k * GS];
             st.L[2 * k] = v.x;
             st.L[2 * k + 1] = v.y;
         });
-        st.mp = X[NP * GS + cc];
+        st.mp = X[NP * GS + cc + 1];
     };
     auto row_of = [&](int t) { return DIR == 0 ? t : Hp - 1 - t; };
     auto load_m = [&](int t, uint4 (&b)[CH]) __attribute__((always_inline)) {
@@ -1671,17 +1676,26 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     uint16_t* const outm = g.out[mode];
     LaneState<CH> sc, sx;
     uint4 mb[CH];
+    // the border slots (columns -1 and nt) of both families
+    for (int i = tid; i < 2 * (NP / 2) * KL; i += blockDim.x) {
+        const int side = i / ((NP / 2) * KL), rem = i % ((NP / 2) * KL);
+        const int pos = side ? nt + 1 : 0, k = rem / KL, lid = rem % KL;
+        ((uint2*)XL)[k * GS + pos * KL + lid] = make_uint2(kGuard2, kGuard2);
+        ((uint2*)XR)[k * GS + pos * KL + lid] = make_uint2(kGuard2, kGuard2);
+        if (k == 0 && lid == 0) {
+            XL[NP * GS + pos] = 0;
+            XR[NP * GS + pos] = 0;
+        }
+    }
     if (t0 > 0) {
         const int pin = (seg - 1) & 1;
-        if (own) st_load(sc, sg_col(pin, 1));
-        if (doFL) {
-            st_load(sx, sg_col(pin, famL));
-            slot_write(XL, sx);
-        }
-        if (doFR) {
-            st_load(sx, sg_col(pin, famR));
-            slot_write(XR, sx);
-        }
+        st_load(sc, sg_col(pin, 1));
+        st_load(sx, sg_col(pin, famL));
+        slot_write(XL, sx);
+        st_load(sx, sg_col(pin, famR));
+        slot_write(XR, sx);
+    } else {
+        vs_guard_all<CH>(sc);
     }
     load_m(t0, mb);
     __syncthreads();
@@ -1701,10 +1715,10 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
 #pragma unroll
                 for (int k = 0; k < 8; k++) m[q][k] = tilde2(m[q][k]);
             });
-            if (own) lane_set<CH, KL>(sc, cg, m, act);
+            lane_set<CH, KL>(sc, cg, m, act);
             lane_set<CH, KL>(sx, cg, m, act);
-            if (doFL) slot_write(XL, sx);
-            if (doFR) slot_write(XR, sx);
+            slot_write(XL, sx);
+            slot_write(XR, sx);
             if (DIR == 1 && keep_sum && g.sum && own && act) edge_sum(y);
             __syncthreads();
             continue;
@@ -1736,10 +1750,10 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         // barrier, its FR reads its own first one, which orders them after
         // this row's writes.
         uint32_t dead[CH][8];
-        if (doFL) slot_read(XL, c - 1, sx);
+        slot_read(XL, c - 1, sx);
         __syncthreads();
-        uint32_t off = 0;   // DEFER: sum of the three pre-step minima
-        if (doFL) {
+        uint32_t off;   // DEFER: sum of the three pre-step minima
+        {
             off = sx.mp;
             lane_step_pk<CH, KL, false, DEFER>(sx, cg, mc, dead);
             if (!wave_act) lane_guard<CH>(sx, act);
@@ -1749,15 +1763,15 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
 #ifndef SISTER_VS_INTERLEAVE
         __builtin_amdgcn_sched_barrier(0);
 #endif
-        if (own) {
+        {
             off += sc.mp;
             lane_step_pk<CH, KL, false, DEFER>(sc, cg, mc, dead);
-            if (edge_row && keep_sum && g.sum && act) edge_out(1, y, sc);
+            if (own && edge_row && keep_sum && g.sum && act) edge_out(1, y, sc);
         }
         __builtin_amdgcn_sched_barrier(0);
-        if (doFR) slot_read(XR, c + 1, sx);
+        slot_read(XR, c + 1, sx);
         __syncthreads();
-        if (doFR) {
+        {
             off += sx.mp;
             lane_step_pk<CH, KL, false, DEFER>(sx, cg, mc, dead);
             if (!wave_act) lane_guard<CH>(sx, act);
@@ -1774,8 +1788,8 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         // hoisting every chunk's LDS reads at once)
         auto chunk_sum = [&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            const uint2* wl = (const uint2*)XL + c * KL + cg.lid + 4 * q * GS;
-            const uint2* wr = (const uint2*)XR + c * KL + cg.lid + 4 * q * GS;
+            const uint2* wl = (const uint2*)XL + (c + 1) * KL + cg.lid + 4 * q * GS;
+            const uint2* wr = (const uint2*)XR + (c + 1) * KL + cg.lid + 4 * q * GS;
             uint32_t lv[8], rv[8];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -2122,7 +2136,7 @@ static int vs_ch(int D) { return ((D + 15) / 16 + kVKL - 1) / kVKL; }
 static size_t vs_lds(int D, int vcols)
 {
     const size_t nt = vcols + 2 * kVGhost;
-    return 2 * (8 * vs_ch(D) * nt * kVKL + nt) * sizeof(uint32_t);
+    return 2 * ((size_t)8 * vs_ch(D) * vs_gs((int)nt) + nt + 2) * sizeof(uint32_t);
 }
 
 int vsweep_pick_cols(long cols, int D, int choice)
