@@ -1175,8 +1175,11 @@ struct FamTune {
 // winner); each chain swept forward (pass 0 direction) then backward.
 template <int CH, int KL, int FAM>
 __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, int nrig, const int* __restrict__ bs,
-                                            int mode, int keep_sum)
+                                            int mode, int keep_sum, int rsplit)
 {
+    // rsplit (rows family before the fused vertical sweep): the backward
+    // direction stores its own (L - M) in part[3] instead of adding the
+    // forward one from part[0], one byte per cell less to move
     extern __shared__ uint32_t lds_s[];   // final sweep: S pairs of the wave's chains
     constexpr int CPW = 64 / KL;          // chains per wave
     constexpr int NO = (FAM == 3) ? 3 : 1;
@@ -1234,7 +1237,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
     LaneState<CH> st;
     for (int dir = 0; dir < 2; dir++) {
         lane_guard<CH>(st, false);
-        const bool need_p = dir == 1;
+        const bool need_p = dir == 1 && !(FAM == 0 && rsplit);
         const bool need_o = final_pass && dir == 1;
         auto fetch = [&](int t, RowBuf<CH, NO>& b) __attribute__((always_inline)) {
             int y, x;
@@ -1302,13 +1305,13 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                         ST16(e, a);
                         ST16(e + 8, bb);
                     } else {
-                        if (dir == 1) {
+                        if (dir == 1 && !rsplit) {
                             uint32_t pf[8];
                             unpack16(b.p[q], pf);
 #pragma unroll
                             for (int i = 0; i < 8; i++) tp[q][i] = W2(U2(tp[q][i]) + U2(pf[i]));
                         }
-                        ST16(part + off + 16 * q, pack16(tp[q]));
+                        ST16((dir == 1 && rsplit ? g.part[3] : part) + off + 16 * q, pack16(tp[q]));
                     }
                 });
                 return;
@@ -1473,7 +1476,9 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 // one segment of kVGhost rows, and the states cross segments through memory
 // (owned columns stored at the end, owned + ghost columns loaded at the start).
 // No block ever waits for another.
-// Buffers: P lives in part[1..2] (u16 per cell, in row planes when FULL), the segment states in part[3]
+// Buffers: P lives in part[1..2] (u16 per cell, in row planes when FULL), the
+// rows family's two directions in part[0] and part[3] (bytes, added as words:
+// each sum is at most 200), the segment states in vstate
 // ([parity][cols, diag, anti][W'][Dp/2 + 4] words: pairs, then minL').
 // ---------------------------------------------------------------------------
 constexpr int kVGhost = 16;        // ghost columns per side = rows per launch
@@ -1565,7 +1570,7 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     uint32_t* XR = xs + NP * GS + nt + 2;
     constexpr int famL = DIR == 0 ? 2 : 3, famR = DIR == 0 ? 3 : 2;   // family ids of FL, FR
     const int SW = Dp / 2 + 4;
-    uint32_t* const Sg = (uint32_t*)g.part[3];
+    uint32_t* const Sg = g.vstate;
     uint16_t* const Pd = (uint16_t*)g.part[1];
     const uint8_t* const msum = g.msum;
     auto sg_col = [&](int par, int f) { return Sg + ((size_t)(par * 3 + f - 1) * Wp + x) * SW; };
@@ -1735,6 +1740,7 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
                 if (!FULL && cg.lid * CH + q >= cg.nq) return;
                 if (DIR == 0) {
                     ob[q] = LD16(g.part[0] + cell + 16 * q);
+                    ob[CH + q] = LD16(g.part[3] + cell + 16 * q);
                 } else {
                     ob[2 * q] = LD16(pd_at(y, q, 0));
                     ob[2 * q + 1] = LD16(pd_at(y, q, 1));
@@ -1812,7 +1818,9 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
                 if (!FULL && cg.lid * CH + q >= cg.nq) return;
                 chunk_sum(qc);
                 uint32_t pr[8];
-                unpack16(ob[q], pr);
+                const uint4 rw = make_uint4(ob[q].x + ob[CH + q].x, ob[q].y + ob[CH + q].y, ob[q].z + ob[CH + q].z,
+                                            ob[q].w + ob[CH + q].w);   // rows fwd + bwd, bytes <= 200
+                unpack16(rw, pr);
 #pragma unroll
                 for (int k = 0; k < 8; k++) acc[q][k] = W2(U2(acc[q][k]) + U2(pr[k]));
                 ST16(pd_at(y, q, 0), make_uint4(acc[q][0], acc[q][1], acc[q][2], acc[q][3]));
@@ -2215,7 +2223,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         for (int gi = 0; gi < ls.ngroups; gi++) {
             const DGroup& G = ls.groups[gi];
             TB_S(kKRows, st);
-            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
+            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 1);
             TE_S(kKRows, st);
             TB_S(kKVSweep, st);
             launch_vsweep(G, mode, keep_sum, st);
@@ -2237,13 +2245,13 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         const DGroup& G = ls.groups[gi];
         if (G.vcols > 0) continue;
         TB_S(kKRows, st);
-        SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
+        SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 0);
         TE_S(kKRows, st);
         TB_S(kKCols, sc);
-        SISTER_FAM(G, 1, dim3(G.blocks[1]), dim3(64), 0, sc, G.rigs, G.nrig, G.bs[1], mode, keep_sum);
+        SISTER_FAM(G, 1, dim3(G.blocks[1]), dim3(64), 0, sc, G.rigs, G.nrig, G.bs[1], mode, keep_sum, 0);
         TE_S(kKCols, sc);
         TB_S(kKDiag, sd);
-        SISTER_FAM(G, 2, dim3(G.blocks[2]), dim3(64), 0, sd, G.rigs, G.nrig, G.bs[2], mode, keep_sum);
+        SISTER_FAM(G, 2, dim3(G.blocks[2]), dim3(64), 0, sd, G.rigs, G.nrig, G.bs[2], mode, keep_sum, 0);
         TE_S(kKDiag, sd);
     }
     if (fs) {
@@ -2256,7 +2264,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         const DGroup& G = ls.groups[gi];
         if (G.vcols > 0) {
             TB_S(kKRows, st);
-            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum);
+            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 1);
             TE_S(kKRows, st);
             TB_S(kKVSweep, st);
             launch_vsweep(G, mode, keep_sum, st);
@@ -2265,7 +2273,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         }
         const size_t lds = (size_t)64 * G.CH[3] * 32;
         TB_S(kKAnti, st);
-        SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[3], mode, keep_sum);
+        SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[3], mode, keep_sum, 0);
         TE_S(kKAnti, st);
     }
 }

@@ -285,6 +285,8 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         g.msum = (uint8_t*)(b.ptr + l.msum);
         for (int q = 0; q < 4; q++) g.part[q] = (uint8_t*)(b.ptr + l.part) + q * align_up(ncell);
         g.edge = (uint16_t*)(b.ptr + l.edge);
+        // 24 W' (Dp/2 + 4) bytes <= 64 W' H' (H' >= 2 D + 4, Dp <= D + 8)
+        g.vstate = (uint32_t*)(b.ptr + l.scratch);
         g.sum = c->keep_sum ? (uint16_t*)(b.ptr + l.sum) : nullptr;
         g.out[0] = r.out_multiview;
         g.out[1] = r.out_horizontal;
