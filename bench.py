@@ -48,13 +48,14 @@ OPS_PER_CELL = {"wta": 28, "msum": 4, "sgm_rows": 16, "sgm_cols": 16, "sgm_diag"
 # (L - M) byte (1), reads it back and writes the two-direction sum (2): 5.
 # The anti-diagonal family, which forms S and the winner, instead reads the
 # three other families' bytes (3) and writes nothing in its backward sweep: 7.
-# The fused vertical sweep (columns, diagonals, anti-diagonals together)
-# reads M (1) and the rows family's byte (1) and writes the u16 partial sum P
-# (2) going down, then reads M (1) and P (2) going up: 7 (against 5 + 5 + 7
-# for the three separate families).  The cost sum over 16x16 tiles writes M
-# once (1); its two-launch form (frames 0/180, then 90/270 read-modify-write)
-# moves 3.
-BYTES_PER_CELL = {"sgm_rows": 5, "sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "sgm_vsweep": 7, "msum": 1}
+# Ahead of the fused vertical sweep the rows family stores its two directions
+# separately (read M twice, write two bytes): 4.  The fused vertical sweep
+# (columns, diagonals, anti-diagonals together) reads M (1) and the rows
+# family's two bytes (2) and writes the u16 partial sum P (2) going down,
+# then reads M (1) and P (2) going up: 8 (against 5 + 5 + 7 for the three
+# separate families).  The cost sum over 16x16 tiles writes M once (1); its
+# two-launch form (frames 0/180, then 90/270 read-modify-write) moves 3.
+BYTES_PER_CELL = {"sgm_rows": 4, "sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "sgm_vsweep": 8, "msum": 1}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch16_rig1080_d192.json")
 
 
