@@ -1519,6 +1519,19 @@ __device__ __forceinline__ uint32_t group_max(uint32_t v)
     return ~group_min<KL>(~v);
 }
 
+// winner key (S << 9) | D of the low (HI = false) or high u16 half S of a pair,
+// in one v_mad_u32_u16 (op_sel picks the half; D <= 64 is an inline constant,
+// 512 comes in a register: one scalar operand at most on this VOP3)
+template <int D, bool HI>
+__device__ __forceinline__ uint32_t vs_key(uint32_t pair, uint32_t m512)
+{
+    static_assert(D >= 0 && D <= 64, "inline constant");
+    uint32_t r;
+    if (HI) asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,0,0,0]" : "=v"(r) : "v"(pair), "v"(m512), "i"(D));
+    else asm("v_mad_u32_u16 %0, %1, %2, %3" : "=v"(r) : "v"(pair), "v"(m512), "i"(D));
+    return r;
+}
+
 template <int CH, int DIR, bool KEEP, bool FULL>
 __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restrict__ rigs, int nrig,
                                                         const int* __restrict__ bs, int ncol, int seg, int mode)
@@ -1806,10 +1819,17 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
             uint32_t mq[8];
             unpack16(mc[q], mq);
             const us2 o2 = DEFER ? U2(dup16(off)) : (us2){0, 0};
+            // going down: sum - 3 M - off; going up S's 8 M is folded in: sum + 5 M - off
+            // (u16 arithmetic modulo 2^16; the true results are below 2^16)
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const us2 m3 = U2(mq[k]) * (us2){3, 3} + o2;
-                acc[q][k] = W2(U2(sc.L[8 * q + k]) + U2(lv[k]) + U2(rv[k]) - m3);
+                if (DIR == 0) {
+                    const us2 m3 = U2(mq[k]) * (us2){3, 3} + o2;
+                    acc[q][k] = W2(U2(sc.L[8 * q + k]) + U2(lv[k]) + U2(rv[k]) - m3);
+                } else {
+                    const us2 m5 = U2(mq[k]) * (us2){5, 5} - o2;
+                    acc[q][k] = W2(U2(sc.L[8 * q + k]) + U2(lv[k]) + U2(rv[k]) + m5);
+                }
             }
         };
         if (DIR == 0) {
@@ -1834,19 +1854,15 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         const bool crop = oy >= 0 && oy < g.H && ox >= 0 && ox < g.W;
         if (!(crop || keep_sum)) continue;
         uint32_t key = 0xFFFFFFFFu;
+        const uint32_t m512 = 512u;
         static_for<0, CH>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const int qg = cg.lid * CH + q;
             chunk_sum(qc);
             const uint32_t pw[8] = {ob[2 * q].x, ob[2 * q].y, ob[2 * q].z, ob[2 * q].w,
                                     ob[2 * q + 1].x, ob[2 * q + 1].y, ob[2 * q + 1].z, ob[2 * q + 1].w};
-            uint32_t mq[8];
-            unpack16(mc[q], mq);
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const us2 s8 = U2(mq[k]) << (us2){3, 3};
-                acc[q][k] = W2(s8 + U2(pw[k]) + U2(acc[q][k]));   // S pairs
-            }
+            for (int k = 0; k < 8; k++) acc[q][k] = W2(U2(pw[k]) + U2(acc[q][k]));   // S pairs
             if (!FULL && qg >= cg.nq) return;
             if (keep_sum && g.sum) {
                 uint16_t* sp = g.sum + cell + 16 * q;
@@ -1858,13 +1874,13 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
             // is added after the minimum (d < 512, no carry into S).  In the
             // half-filled last chunk (D % 16 == 8) pairs 4..7 are padding.
             const bool tail = cg.half_tail && qg == cg.nq - 1;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const uint32_t k0 = ((acc[q][k] & 0xFFFFu) << 9) | (uint32_t)(16 * q + 2 * k);
-                const uint32_t k1 = ((acc[q][k] >> 16) << 9) | (uint32_t)(16 * q + 2 * k + 1);
+            static_for<0, 8>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                const uint32_t k0 = vs_key<16 * q + 2 * k, false>(acc[q][k], m512);
+                const uint32_t k1 = vs_key<16 * q + 2 * k + 1, true>(acc[q][k], m512);
                 const uint32_t kk = min(k0, k1);
                 key = (k < 4 || !tail) ? min(key, kk) : key;
-            }
+            });
             __builtin_amdgcn_sched_barrier(0);
         });
         if (!crop) continue;
