@@ -955,10 +955,12 @@ __global__ __launch_bounds__(1024) void k_msum_tile(const RigDesc* __restrict__ 
 //   - r1 / r3 entering from beyond the left/right border see L' = guard and
 //     minL' = 0, so L = M + P2 (the standard step from a guard state).
 // Each family stores, for rows 1..H'-2, (L_fwd - M) + (L_bwd - M) in [0, 200]
-// as a byte (part[f]), and for rows 0 and H'-1 its exact contribution as u16
-// (edge[f]).  The anti-diagonal family runs last: its upward sweep forms
-//   S = 8 M + sum_f part[f]      (rows 1..H'-2)
-// and takes the winner (hpp:283) for cells inside the crop.
+// as a byte (part[f]; the rows family its two directions as separate bytes,
+// part[0] and rowsb, so they run independently), and for rows 0 and H'-1 its
+// exact contribution as u16 (edge[f]).  The anti-diagonal family runs last:
+// its upward sweep forms
+//   S = 8 M + part[0] + rowsb + part[1] + part[2] + its own (L - M) sum
+// (rows 1..H'-2) and takes the winner (hpp:283) for cells inside the crop.
 // ---------------------------------------------------------------------------
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 constexpr uint32_t kGuard2 = 0x3FFF3FFFu;
@@ -1158,16 +1160,23 @@ __device__ __forceinline__ uint4 add_u16x8(uint4 a, uint4 b)
 }
 
 // prefetch depth of the family kernels (cells whose loads are in flight per
-// chain); overridable at build time for tuning
+// chain); overridable at build time for tuning.  At 16 lanes per chain the
+// rows, columns and diagonals take 8 (a single rig has under one wave per
+// SIMD, so the bytes in flight per wave set its bandwidth); the
+// anti-diagonal family, whose cells also carry three other families' bytes,
+// keeps 4 (8 would double its registers and halve its occupancy).
 #ifndef SISTER_PF16
-#define SISTER_PF16 4
+#define SISTER_PF16 8
+#endif
+#ifndef SISTER_PF16A
+#define SISTER_PF16A 4
 #endif
 #ifndef SISTER_PF8
 #define SISTER_PF8 2
 #endif
 template <int CH, int KL, int FAM>
 struct FamTune {
-    static constexpr int PF = KL == 16 ? SISTER_PF16 : (KL == 8 ? SISTER_PF8 : 2);
+    static constexpr int PF = KL == 16 ? (FAM == 3 ? SISTER_PF16A : SISTER_PF16) : (KL == 8 ? SISTER_PF8 : 2);
 };
 
 // Family kernel: FAM 0 rows (lanes over rows, steps along x), 1 columns,
@@ -1177,13 +1186,22 @@ template <int CH, int KL, int FAM>
 __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, int nrig, const int* __restrict__ bs,
                                             int mode, int keep_sum, int rsplit)
 {
-    // rsplit (rows family before the fused vertical sweep): the backward
-    // direction stores its own (L - M) in part[3] instead of adding the
-    // forward one from part[0], one byte per cell less to move
+    // The rows family stores its two directions' (L - M) separately (forward
+    // in part[0], backward in rowsb), so neither reads the other.  rsplit: the
+    // two directions run as separate waves, the grid doubled and its second
+    // half sweeping backward; a wave holding the top or bottom row's chain,
+    // whose exact edge values add both directions, sweeps both in its
+    // first-half block.
     extern __shared__ uint32_t lds_s[];   // final sweep: S pairs of the wave's chains
     constexpr int CPW = 64 / KL;          // chains per wave
-    constexpr int NO = (FAM == 3) ? 3 : 1;
-    const int ri = find_rig(bs, nrig, blockIdx.x);
+    constexpr int NO = (FAM == 3) ? 4 : 1;
+    int bid = blockIdx.x, dsel = -1;      // dsel: the one direction of this block, -1 both
+    if (FAM == 0 && rsplit) {
+        const int half = bs[nrig];
+        dsel = bid >= half ? 1 : 0;
+        bid -= dsel * half;
+    }
+    const int ri = find_rig(bs, nrig, bid);
     const RigDesc& g = rigs[ri];
     const int lane = threadIdx.x;
     const int grp = lane / KL;
@@ -1192,10 +1210,10 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
     cg.nq = g.Dp >> 4;
     cg.half_tail = (g.D & 15) != 0;
     const int Wp = g.Wp, Hp = g.Hp, Dp = g.Dp, D = g.D;
-    const int c = (blockIdx.x - bs[ri]) * CPW + grp;     // chain index
+    const int c = (bid - bs[ri]) * CPW + grp;     // chain index
     const int nchain = FAM == 0 ? Hp : (FAM == 1 ? Wp : Wp + Hp - 1);
     const bool valid = c < nchain;
-    const int cw0 = (blockIdx.x - bs[ri]) * CPW;
+    const int cw0 = (bid - bs[ri]) * CPW;
     const int cw1 = min(cw0 + CPW - 1, nchain - 1);
     uint8_t* part = g.part[FAM];
     uint16_t* edge = g.edge + (size_t)FAM * 2 * (size_t)Wp * Dp;
@@ -1234,10 +1252,19 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
     const bool want_sub = final_pass && mode == 0 && g.subpix != nullptr;
     uint16_t* outm = g.out[mode];
 
+    int dlo = 0, dhi = 1;
+    if (dsel >= 0) {
+        const bool edge_wave = cw0 == 0 || cw1 >= Hp - 1;
+        if (edge_wave) {
+            if (dsel == 1) return;
+        } else {
+            dlo = dhi = dsel;
+        }
+    }
     LaneState<CH> st;
-    for (int dir = 0; dir < 2; dir++) {
+    for (int dir = dlo; dir <= dhi; dir++) {
         lane_guard<CH>(st, false);
-        const bool need_p = dir == 1 && !(FAM == 0 && rsplit);
+        const bool need_p = dir == 1 && FAM != 0;
         const bool need_o = final_pass && dir == 1;
         auto fetch = [&](int t, RowBuf<CH, NO>& b) __attribute__((always_inline)) {
             int y, x;
@@ -1251,9 +1278,10 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                 if (need_p) b.p[q] = ok ? LD16(part + off + 16 * q) : make_uint4(0, 0, 0, 0);
                 if (need_o) {
                     b.o[q] = ok ? LD16(g.part[0] + off + 16 * q) : make_uint4(0, 0, 0, 0);
-                    if (NO == 3) {
+                    if (NO == 4) {
                         b.o[CH + q] = ok ? LD16(g.part[1] + off + 16 * q) : make_uint4(0, 0, 0, 0);
                         b.o[2 * CH + q] = ok ? LD16(g.part[2] + off + 16 * q) : make_uint4(0, 0, 0, 0);
+                        b.o[3 * CH + q] = ok ? LD16(g.rowsb + off + 16 * q) : make_uint4(0, 0, 0, 0);
                     }
                 }
             });
@@ -1305,13 +1333,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                         ST16(e, a);
                         ST16(e + 8, bb);
                     } else {
-                        if (dir == 1 && !rsplit) {
-                            uint32_t pf[8];
-                            unpack16(b.p[q], pf);
-#pragma unroll
-                            for (int i = 0; i < 8; i++) tp[q][i] = W2(U2(tp[q][i]) + U2(pf[i]));
-                        }
-                        ST16((dir == 1 && rsplit ? g.part[3] : part) + off + 16 * q, pack16(tp[q]));
+                        ST16((dir == 1 ? g.rowsb : part) + off + 16 * q, pack16(tp[q]));
                     }
                 });
                 return;
@@ -1391,9 +1413,10 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                 if (!(crop || keep_sum)) return;
                 // S = 8 M + part[rows] + part[cols] + part[diag] + this family
                 uint32_t pr[8], pc[8], pd[8], S[8];
-                unpack16(b.o[q], pr);
-                unpack16(b.o[NO == 3 ? CH + q : q], pc);
-                unpack16(b.o[NO == 3 ? 2 * CH + q : q], pd);
+                const uint4 rf = b.o[q], rb = b.o[NO == 4 ? 3 * CH + q : q];
+                unpack16(make_uint4(rf.x + rb.x, rf.y + rb.y, rf.z + rb.z, rf.w + rb.w), pr);   // bytes <= 200
+                unpack16(b.o[NO == 4 ? CH + q : q], pc);
+                unpack16(b.o[NO == 4 ? 2 * CH + q : q], pd);
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     const us2 s8 = U2(m[q][i]) << (us2){3, 3};
@@ -1477,8 +1500,8 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 // (owned columns stored at the end, owned + ghost columns loaded at the start).
 // No block ever waits for another.
 // Buffers: P lives in part[1..2] (u16 per cell, in row planes when FULL), the
-// rows family's two directions in part[0] and part[3] (bytes, added as words:
-// each sum is at most 200), the segment states in vstate
+// rows family's two directions in part[0] and rowsb (= part[3] here; bytes,
+// added as words: each sum is at most 200), the segment states in vstate
 // ([parity][cols, diag, anti][W'][Dp/2 + 4] words: pairs, then minL').
 // ---------------------------------------------------------------------------
 constexpr int kVGhost = 16;        // ghost columns per side = rows per launch
@@ -1753,7 +1776,7 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
                 if (!FULL && cg.lid * CH + q >= cg.nq) return;
                 if (DIR == 0) {
                     ob[q] = LD16(g.part[0] + cell + 16 * q);
-                    ob[CH + q] = LD16(g.part[3] + cell + 16 * q);
+                    ob[CH + q] = LD16(g.rowsb + cell + 16 * q);
                 } else {
                     ob[2 * q] = LD16(pd_at(y, q, 0));
                     ob[2 * q + 1] = LD16(pd_at(y, q, 1));
@@ -2239,7 +2262,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         for (int gi = 0; gi < ls.ngroups; gi++) {
             const DGroup& G = ls.groups[gi];
             TB_S(kKRows, st);
-            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 1);
+            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 0);
             TE_S(kKRows, st);
             TB_S(kKVSweep, st);
             launch_vsweep(G, mode, keep_sum, st);
@@ -2261,7 +2284,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         const DGroup& G = ls.groups[gi];
         if (G.vcols > 0) continue;
         TB_S(kKRows, st);
-        SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 0);
+        SISTER_FAM(G, 0, dim3(2 * G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 1);
         TE_S(kKRows, st);
         TB_S(kKCols, sc);
         SISTER_FAM(G, 1, dim3(G.blocks[1]), dim3(64), 0, sc, G.rigs, G.nrig, G.bs[1], mode, keep_sum, 0);
@@ -2280,7 +2303,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         const DGroup& G = ls.groups[gi];
         if (G.vcols > 0) {
             TB_S(kKRows, st);
-            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 1);
+            SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 0);
             TE_S(kKRows, st);
             TB_S(kKVSweep, st);
             launch_vsweep(G, mode, keep_sum, st);

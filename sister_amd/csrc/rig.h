@@ -35,9 +35,11 @@ struct RigDesc {
     uint8_t* msum;            // mask-weighted cost sum M, H' x W' x Dp (u8: M <= 4*63)
     uint8_t* part[4];         // per chain family (rows, cols, diag, anti): for rows
                               // 1..H'-2 the two directions' (L - M) summed, in [0,200];
-                              // with the fused vertical sweep, part[0] / part[3] hold the
-                              // rows family's forward / backward (L - M), and part[1..2]
-                              // the sweep's u16 partial sums
+                              // the rows family keeps its forward (L - M) here and its
+                              // backward one in rowsb; with the fused vertical sweep
+                              // part[1..2] hold the sweep's u16 partial sums
+    uint8_t* rowsb;           // rows family's backward (L - M), rows 1..H'-2: part[3] ahead
+                              // of the fused vertical sweep, a volume of its own otherwise
     uint32_t* vstate;         // fused vertical sweep: segment-end states
                               // [2][3][W'][Dp/2 + 4] (the median scratch, free by then)
     uint16_t* edge;           // exact family contributions on rows 0 and H'-1:

@@ -36,13 +36,16 @@ namespace {
 
 // workspace layout of one rig (offsets from the workspace base)
 struct WsLayout {
-    size_t code, wta, med, mask, msum, part, edge, scratch, sum, total;
+    size_t code, wta, med, mask, msum, part, edge, scratch, sum, rowsb, total;
 };
 
 inline int padded_depth(int D) { return (D + 15) & ~15; }
 
-// keep_sum adds the debug copy of S
-WsLayout ws_layout(int W, int H, int D, bool keep_sum = false)
+// keep_sum adds the debug copy of S; rows_b the rows family's backward bytes
+// of the separate chain families (the fused vertical sweep keeps them in part
+// volume 3).  Footprints for admission and sister_workspace_bytes take the
+// default, the larger of the two.
+WsLayout ws_layout(int W, int H, int D, bool keep_sum = false, bool rows_b = true)
 {
     const size_t Wp = (size_t)W + 2 * D, Hp = (size_t)H + 2 * D;
     const size_t npx = Wp * Hp, ncell = npx * padded_depth(D);
@@ -57,6 +60,7 @@ WsLayout ws_layout(int W, int H, int D, bool keep_sum = false)
     l.edge = o;    o += align_up(4 * 2 * Wp * padded_depth(D) * sizeof(uint16_t));
     l.scratch = o; o += align_up(8 * npx * sizeof(uint64_t));
     l.sum = o;     if (keep_sum) o += align_up(ncell * sizeof(uint16_t));
+    l.rowsb = o;   if (rows_b) o += align_up(ncell);
     l.total = o;
     return l;
 }
@@ -266,28 +270,6 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         g.Dp = padded_depth(g.D);
         g.stride = r.stride;
         for (int v = 0; v < 5; v++) g.view[v] = r.view[v];
-        WsLayout l = ws_layout(g.W, g.H, g.D, c->keep_sum);
-        Block b;
-        if (c->pool.get(l.total, b) != 0) {
-            for (auto& w : p.workspaces) c->pool.put(w);
-            p.workspaces.clear();
-            return fail(c, SISTER_ENOMEM, "workspace of " + std::to_string(l.total) + " bytes exceeds the HBM budget");
-        }
-        p.workspaces.push_back(b);
-        const size_t npx = (size_t)g.Wp * g.Hp;
-        for (int q = 0; q < 8; q++) {
-            g.code[q] = (uint64_t*)(b.ptr + l.code) + q * npx;
-            g.wta[q] = (uint16_t*)(b.ptr + l.wta) + q * npx;
-            g.med[q] = (uint16_t*)(b.ptr + l.med) + q * npx;
-        }
-        for (int q = 0; q < 4; q++) g.mask[q] = (uint8_t*)(b.ptr + l.mask) + q * npx;
-        const size_t ncell = npx * g.Dp;
-        g.msum = (uint8_t*)(b.ptr + l.msum);
-        for (int q = 0; q < 4; q++) g.part[q] = (uint8_t*)(b.ptr + l.part) + q * align_up(ncell);
-        g.edge = (uint16_t*)(b.ptr + l.edge);
-        // 24 W' (Dp/2 + 4) bytes <= 64 W' H' (H' >= 2 D + 4, Dp <= D + 8)
-        g.vstate = (uint32_t*)(b.ptr + l.scratch);
-        g.sum = c->keep_sum ? (uint16_t*)(b.ptr + l.sum) : nullptr;
         g.out[0] = r.out_multiview;
         g.out[1] = r.out_horizontal;
         g.out[2] = r.out_vertical;
@@ -314,6 +296,37 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         groups.push_back(G);
         k = e;
     }
+    // workspaces: a rig of a fused group keeps the rows family's backward
+    // bytes in part volume 3, the others in a volume of their own
+    std::vector<WsLayout> lay(n);
+    for (const GroupHost& G : groups)
+        for (int k = G.i0; k < G.i1; k++) {
+            RigDesc& g = p.host_rigs[k];
+            const WsLayout l = ws_layout(g.W, g.H, g.D, c->keep_sum, G.vcols == 0);
+            lay[k] = l;
+            Block b;
+            if (c->pool.get(l.total, b) != 0) {
+                for (auto& w : p.workspaces) c->pool.put(w);
+                p.workspaces.clear();
+                return fail(c, SISTER_ENOMEM, "workspace of " + std::to_string(l.total) + " bytes exceeds the HBM budget");
+            }
+            p.workspaces.push_back(b);
+            const size_t npx = (size_t)g.Wp * g.Hp;
+            for (int q = 0; q < 8; q++) {
+                g.code[q] = (uint64_t*)(b.ptr + l.code) + q * npx;
+                g.wta[q] = (uint16_t*)(b.ptr + l.wta) + q * npx;
+                g.med[q] = (uint16_t*)(b.ptr + l.med) + q * npx;
+            }
+            for (int q = 0; q < 4; q++) g.mask[q] = (uint8_t*)(b.ptr + l.mask) + q * npx;
+            const size_t ncell = npx * g.Dp;
+            g.msum = (uint8_t*)(b.ptr + l.msum);
+            for (int q = 0; q < 4; q++) g.part[q] = (uint8_t*)(b.ptr + l.part) + q * align_up(ncell);
+            g.rowsb = G.vcols > 0 ? g.part[3] : (uint8_t*)(b.ptr + l.rowsb);
+            g.edge = (uint16_t*)(b.ptr + l.edge);
+            // 24 W' (Dp/2 + 4) bytes <= 64 W' H' (H' >= 2 D + 4, Dp <= D + 8)
+            g.vstate = (uint32_t*)(b.ptr + l.scratch);
+            g.sum = c->keep_sum ? (uint16_t*)(b.ptr + l.sum) : nullptr;
+        }
     if ((int)groups.size() > kMaxDGroups) {
         for (auto& w : p.workspaces) c->pool.put(w);
         p.workspaces.clear();
@@ -331,10 +344,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
     uint64_t** scr = (uint64_t**)(host.data() + rig_bytes);
     int* bs = (int*)(host.data() + rig_bytes + scr_bytes);
     int* gbs = (int*)(host.data() + rig_bytes + scr_bytes + bs_bytes);
-    for (int k = 0; k < n; k++) {
-        WsLayout l = ws_layout(p.host_rigs[k].W, p.host_rigs[k].H, p.host_rigs[k].D, c->keep_sum);
-        scr[k] = (uint64_t*)(p.workspaces[k].ptr + l.scratch);
-    }
+    for (int k = 0; k < n; k++) scr[k] = (uint64_t*)(p.workspaces[k].ptr + lay[k].scratch);
     for (int st = 0; st < kNumStages; st++) p.ls.blocks[st] = 0;
     const bool wta_rows = wta_rows_ok(maxCols, maxD);
     for (int k = 0; k < n; k++) {
