@@ -1483,7 +1483,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 // of a pass in one sweep (same recurrence and pass rules as k_fam).  Lanes own
 // columns (kVKL = 4 lanes per column, CH chunks of 16 disparities each), so
 // every cell of a row sees its three vertical paths in one lane group:
-//   downward pass:  P = part[rows] + sum_f (L_f,down - M)  (u16, rows 1..H'-2)
+//   downward pass:  P = rows_fwd + rows_bwd + sum_f (L_f,down - M)  (u16, rows 1..H'-2)
 //   upward pass:    S = 8 M + P + sum_f (L_f,up - M), winner and sub-pixel
 // -- 8 bytes per cell over both passes, against 17 for the three separate
 // families (5 + 5 + 7).  Along a pass the diagonal-family state moves one
@@ -1492,9 +1492,9 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 // barriers.  Call FL the family whose state moves to higher columns (diagonal
 // going down, anti-diagonal going up) and FR the other.
 // A block owns ncol columns and also computes kVGhost ghost columns on each
-// side (one wave each: FL only on the left, FR only on the right, the only
-// families that flow inwards).  A wrong state entering at the ghost edge
-// moves inwards one column per row, so the owned columns stay exact for
+// side (one wave each; what matters there is the family that flows inwards,
+// FL on the left and FR on the right).  A wrong state entering at the ghost
+// edge moves inwards one column per row, so the owned columns stay exact for
 // kVGhost rows without any exchange with neighbouring blocks: one launch runs
 // one segment of kVGhost rows, and the states cross segments through memory
 // (owned columns stored at the end, owned + ghost columns loaded at the start).
@@ -1605,11 +1605,16 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     uint32_t* XL = xs;
     uint32_t* XR = xs + NP * GS + nt + 2;
     constexpr int famL = DIR == 0 ? 2 : 3, famR = DIR == 0 ? 3 : 2;   // family ids of FL, FR
-    const int SW = Dp / 2 + 4;
-    uint32_t* const Sg = g.vstate;
+    // segment states as bytes: at the segment's last row r a chain's
+    // t = L - M(r) lies in [m, m + P2] (m = its row r-1 minimum, reached at that
+    // row's argmin), so t - min_d t is the normalised (L - M) in [0, 100].  A
+    // column's state is those Dp bytes and the u32 min_d L - min_d t, SB bytes
+    // in all; the next segment rebuilds the normalised L = M(r) + byte.
+    const int SB = Dp + 16;
+    uint8_t* const Sg = (uint8_t*)g.vstate;
     uint16_t* const Pd = (uint16_t*)g.part[1];
     const uint8_t* const msum = g.msum;
-    auto sg_col = [&](int par, int f) { return Sg + ((size_t)(par * 3 + f - 1) * Wp + x) * SW; };
+    auto sg_col = [&](int par, int f) { return Sg + ((size_t)(par * 3 + f - 1) * Wp + x) * SB; };
     // P is this kernel's own intermediate.  FULL: planes per (row, chunk slot
     // q, half h) in which lane (x, lid) holds 8 u16 at byte (x KL + lid) * 16,
     // so a wave's 64 lanes move 1 KB contiguous per access (3 ms faster than
@@ -1625,33 +1630,48 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     };
 
     // lane state <-> a column vector in memory (pairs, then minL' at word Dp/2)
-    auto st_load = [&](LaneState<CH>& st, const uint32_t* v) __attribute__((always_inline)) {
+    // mr: the cost bytes M(r) of the segment's last row r (the previous
+    // segment's, when loading); padding pairs come back as guards
+    auto st_load = [&](LaneState<CH>& st, const uint8_t* v, const uint4 (&mr)[CH]) __attribute__((always_inline)) {
         static_for<0, CH>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const int qg = cg.lid * CH + q;
-            uint4 a = make_uint4(kGuard2, kGuard2, kGuard2, kGuard2), b = a;
-            if (act && (FULL || qg < cg.nq)) {
-                a = LD16(v + qg * 8);
-                b = LD16(v + qg * 8 + 4);
-            }
-            st.L[8 * q + 0] = a.x; st.L[8 * q + 1] = a.y; st.L[8 * q + 2] = a.z; st.L[8 * q + 3] = a.w;
-            st.L[8 * q + 4] = b.x; st.L[8 * q + 5] = b.y; st.L[8 * q + 6] = b.z; st.L[8 * q + 7] = b.w;
+            const bool ok = act && (FULL || qg < cg.nq);
+            const bool tail = !FULL && cg.half_tail && qg == cg.nq - 1;
+            uint32_t m[8], bt[8];
+            unpack16(mr[q], m);
+            unpack16(ok ? LD16(v + qg * 16) : make_uint4(0, 0, 0, 0), bt);
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                st.L[8 * q + k] = (!ok || (tail && k >= 4)) ? kGuard2 : W2(U2(m[k]) + U2(bt[k]));
         });
-        st.mp = act ? G(v)[Dp / 2] : 0u;
+        st.mp = act ? G((const uint32_t*)(v + Dp))[0] : 0u;
     };
-    auto st_store = [&](const LaneState<CH>& st, uint32_t* v) __attribute__((always_inline)) {
+    auto st_store = [&](const LaneState<CH>& st, uint8_t* v, const uint4 (&mr)[CH]) __attribute__((always_inline)) {
+        uint32_t tq[CH][8];
+        us2 tm = U2(0xFFFFFFFFu);
+        static_for<0, CH>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            uint32_t m[8];
+            unpack16(mr[q], m);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                tq[q][k] = W2(U2(st.L[8 * q + k]) - U2(m[k]));   // padding pairs: guard-sized
+                tm = umin2(tm, U2(tq[q][k]));
+            }
+        });
+        const uint32_t tmin = group_min<KL>(min((uint32_t)tm.x, (uint32_t)tm.y));
+        const us2 n = U2(dup16(tmin));
         static_for<0, CH>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const int qg = cg.lid * CH + q;
-            if (FULL || qg < cg.nq) {
-                // DEFER: stored minus minL' (guard pairs stay above 0x3FFF - 5000)
-                const us2 n = DEFER ? U2(dup16(st.mp)) : (us2){0, 0};
-                auto w = [&](int k) __attribute__((always_inline)) { return W2(U2(st.L[8 * q + k]) - n); };
-                ST16(v + qg * 8, make_uint4(w(0), w(1), w(2), w(3)));
-                ST16(v + qg * 8 + 4, make_uint4(w(4), w(5), w(6), w(7)));
-            }
+            if (!(FULL || qg < cg.nq)) return;
+            uint32_t b[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) b[k] = W2(U2(tq[q][k]) - n);
+            ST16(v + qg * 16, pack16(b));
         });
-        if (cg.lid == 0) G(v)[Dp / 2] = DEFER ? 0u : st.mp;
+        if (cg.lid == 0) G((uint32_t*)(v + Dp))[0] = st.mp - tmin;
     };
     auto slot_write = [&](uint32_t* X, const LaneState<CH>& st) __attribute__((always_inline)) {
         uint2* w = (uint2*)X + (c + 1) * KL + cg.lid;
@@ -1730,10 +1750,12 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     }
     if (t0 > 0) {
         const int pin = (seg - 1) & 1;
-        st_load(sc, sg_col(pin, 1));
-        st_load(sx, sg_col(pin, famL));
+        uint4 mr[CH];
+        load_m(t0 - 1, mr);
+        st_load(sc, sg_col(pin, 1), mr);
+        st_load(sx, sg_col(pin, famL), mr);
         slot_write(XL, sx);
-        st_load(sx, sg_col(pin, famR));
+        st_load(sx, sg_col(pin, famR), mr);
         slot_write(XR, sx);
     } else {
         vs_guard_all<CH>(sc);
@@ -1802,9 +1824,7 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
             slot_write(XL, sx);
             if (own && edge_row && keep_sum && g.sum && act) edge_out(famL, y, sx);
         }
-#ifndef SISTER_VS_INTERLEAVE
         __builtin_amdgcn_sched_barrier(0);
-#endif
         {
             off += sc.mp;
             lane_step_pk<CH, KL, false, DEFER>(sc, cg, mc, dead);
@@ -1937,11 +1957,13 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     // states at the end of the segment, for the next launch
     if (t0 + nrows < Hp && own && act) {
         const int pout = seg & 1;
-        st_store(sc, sg_col(pout, 1));
+        uint4 mr[CH];
+        load_m(t0 + nrows - 1, mr);
+        st_store(sc, sg_col(pout, 1), mr);
         slot_read(XL, c, sx);
-        st_store(sx, sg_col(pout, famL));
+        st_store(sx, sg_col(pout, famL), mr);
         slot_read(XR, c, sx);
-        st_store(sx, sg_col(pout, famR));
+        st_store(sx, sg_col(pout, famR), mr);
     }
 }
 

@@ -40,8 +40,9 @@ struct RigDesc {
                               // part[1..2] hold the sweep's u16 partial sums
     uint8_t* rowsb;           // rows family's backward (L - M), rows 1..H'-2: part[3] ahead
                               // of the fused vertical sweep, a volume of its own otherwise
-    uint32_t* vstate;         // fused vertical sweep: segment-end states
-                              // [2][3][W'][Dp/2 + 4] (the median scratch, free by then)
+    uint32_t* vstate;         // fused vertical sweep: segment-end states, normalised
+                              // (L - M) bytes and minL': [2][3][W'][Dp + 16] bytes
+                              // (the median scratch, free by then)
     uint16_t* edge;           // exact family contributions on rows 0 and H'-1:
                               // [family][2][W'][Dp]
     uint16_t* sum;            // SGM aggregate S, H' x W' x Dp (debug only, may be null)

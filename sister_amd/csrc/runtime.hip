@@ -323,7 +323,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
             for (int q = 0; q < 4; q++) g.part[q] = (uint8_t*)(b.ptr + l.part) + q * align_up(ncell);
             g.rowsb = G.vcols > 0 ? g.part[3] : (uint8_t*)(b.ptr + l.rowsb);
             g.edge = (uint16_t*)(b.ptr + l.edge);
-            // 24 W' (Dp/2 + 4) bytes <= 64 W' H' (H' >= 2 D + 4, Dp <= D + 8)
+            // 6 W' (Dp + 16) bytes <= 64 W' H' (H' >= 2 D + 4, Dp <= D + 8)
             g.vstate = (uint32_t*)(b.ptr + l.scratch);
             g.sum = c->keep_sum ? (uint16_t*)(b.ptr + l.sum) : nullptr;
         }
