@@ -1620,7 +1620,11 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     // so a wave's 64 lanes move 1 KB contiguous per access (3 ms faster than
     // the cell layout at 16 rigs); otherwise cell layout (chunk qg of cell
     // (y, x) at (y W' + x) Dp + 16 qg).
-    const uint32_t pd_lane = (uint32_t)(x * KL + cg.lid) * 16u;
+    // FULL: loads of a column outside the image read the nearest inside one
+    // (in bounds, no exec-masked loads or zero fills in the row loop); such a
+    // column's states are reset to guards and nothing of it is stored
+    const int xc = min(max(x, 0), Wp - 1);
+    const uint32_t pd_lane = (uint32_t)(xc * KL + cg.lid) * 16u;
     auto pd_at = [&](int y, int q, int h) __attribute__((always_inline)) {
         if (FULL) {
             const uint16_t* plane = Pd + (((size_t)y * CH + q) * 2 + h) * ((size_t)Wp * KL * 8);   // uniform
@@ -1693,11 +1697,11 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     };
     auto row_of = [&](int t) { return DIR == 0 ? t : Hp - 1 - t; };
     auto load_m = [&](int t, uint4 (&b)[CH]) __attribute__((always_inline)) {
-        const size_t cell = ((size_t)row_of(t) * Wp + x) * Dp + boff;
+        const size_t cell = ((size_t)row_of(t) * Wp + (FULL ? xc : x)) * Dp + boff;
         static_for<0, CH>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const bool ok = act && (FULL || cg.lid * CH + q < cg.nq);
-            b[q] = ok ? LD16(msum + cell + 16 * q) : make_uint4(0, 0, 0, 0);
+            b[q] = FULL ? LD16(msum + cell + 16 * q) : ok ? LD16(msum + cell + 16 * q) : make_uint4(0, 0, 0, 0);
         });
     };
     // exact L of a family on an edge row (debug sums only), as k_fam stores it
