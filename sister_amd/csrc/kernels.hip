@@ -451,13 +451,34 @@ __global__ __launch_bounds__(256) void k_median_prep(const RigDesc* __restrict__
     out[q] = (uint64_t)v1 | ((uint64_t)v2 << 16) | ((uint64_t)v3 << 32) | ((uint64_t)v4 << 48);
 }
 
-__device__ __forceinline__ unsigned med9_pre(uint64_t s, unsigned a, unsigned b, unsigned c)
+__device__ __forceinline__ unsigned med9_4(unsigned s1, unsigned s2, unsigned s3, unsigned s4, unsigned a,
+                                           unsigned b, unsigned c)
 {
     cswap(a, b); cswap(b, c); cswap(a, b);
-    const unsigned s1 = (unsigned)(s & 0xFFFF), s2 = (unsigned)((s >> 16) & 0xFFFF);
-    const unsigned s3 = (unsigned)((s >> 32) & 0xFFFF), s4 = (unsigned)(s >> 48);
     return min(min(s4, max(s3, a)), min(max(s2, b), max(s1, c)));
 }
+
+__device__ __forceinline__ unsigned med9_pre(uint64_t s, unsigned a, unsigned b, unsigned c)
+{
+    return med9_4((unsigned)(s & 0xFFFF), (unsigned)((s >> 16) & 0xFFFF), (unsigned)((s >> 32) & 0xFFFF),
+                  (unsigned)(s >> 48), a, b, c);
+}
+
+// the same from the six fixed values themselves (12-comparator network)
+__device__ __forceinline__ unsigned med9_raw(const uint32_t (&w)[6], unsigned a, unsigned b, unsigned c)
+{
+    unsigned v0 = w[0], v1 = w[1], v2 = w[2], v3 = w[3], v4 = w[4], v5 = w[5];
+    cswap(v0, v5); cswap(v1, v3); cswap(v2, v4);
+    cswap(v1, v2); cswap(v3, v4);
+    cswap(v0, v3); cswap(v2, v5);
+    cswap(v0, v1); cswap(v2, v3); cswap(v4, v5);
+    cswap(v1, v2); cswap(v3, v4);
+    return med9_4(v1, v2, v3, v4, a, b, c);
+}
+
+template <int K, bool RAW> struct MedBuf;
+template <int K> struct MedBuf<K, true> { uint32_t v[K][6]; uint32_t l[6]; };   // six fixed values
+template <int K> struct MedBuf<K, false> { uint64_t v[K]; uint64_t l; };        // their s1..s4
 
 // One block per (rig, map), 1024 threads; rows in sequence, columns in
 // parallel: thread t owns columns t + 1024 k for k < K = ceil(C / 1024), and
@@ -474,9 +495,13 @@ __device__ __forceinline__ unsigned med9_pre(uint64_t s, unsigned a, unsigned b,
 // slot K - 1 can reach past column C - 2.  PF: load row r + 1 during row r
 // (frames up to 8192 wide); without it the loads of a row precede its
 // compute, which keeps the 16-slot buffer of the widest frames in registers.
+// RAW (frames up to 4096 wide): the six fixed values of each position are
+// loaded a row ahead and sorted in the row's step, so no prepass runs; wider
+// frames read the prepass's order statistics (fewer registers per slot).
 constexpr int kMedMaxK = 16;
+constexpr int kMedRawK = 4;
 
-template <int K, bool EXACT, bool PF>
+template <int K, bool EXACT, bool PF, bool RAW>
 __device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t* __restrict__ in,
                                                    uint16_t* __restrict__ out,
                                                    const uint64_t* __restrict__ pre, uint16_t* lds)
@@ -509,20 +534,36 @@ __device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t*
     auto at = [&](int i) __attribute__((always_inline)) {
         return G((const uint64_t*)((const char*)pre + (uint32_t)i * 8u));
     };
-    auto fetch = [&](int r, uint64_t (&nx)[K], uint64_t& nlast) __attribute__((always_inline)) {
+    // RAW: in[q-1..q+1] and in[q+C-1..q+C+1] at q clamped to qhi (C <= q:
+    // every index in bounds; the clamped positions' values go unused)
+    auto six = [&](int q, uint32_t (&o)[6]) __attribute__((always_inline)) {
+        const int qq = min(q, qhi);
+        o[0] = G(in)[qq - 1]; o[1] = G(in)[qq]; o[2] = G(in)[qq + 1];
+        o[3] = G(in)[qq + C - 1]; o[4] = G(in)[qq + C]; o[5] = G(in)[qq + C + 1];
+    };
+    auto fetch = [&](int r, MedBuf<K, RAW>& b) __attribute__((always_inline)) {
         const int base = r * C;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             if (!EXACT && k >= kact) break;
-            nx[k] = *at(min(base + col(k), qhi));
+            if constexpr (RAW) six(base + col(k), b.v[k]);
+            else b.v[k] = *at(min(base + col(k), qhi));
         }
-        if (wave0) nlast = *at(min(base + C - 1 + tid, qhi));
+        if constexpr (RAW) {
+            if (wave0) six(base + C - 1 + tid, b.l);
+        } else {
+            if (wave0) b.l = *at(min(base + C - 1 + tid, qhi));
+        }
+    };
+    auto med = [&](const MedBuf<K, RAW>& b, int k, unsigned a, unsigned bb, unsigned c) __attribute__((always_inline)) {
+        if constexpr (RAW) return k < 0 ? med9_raw(b.l, a, bb, c) : med9_raw(b.v[k], a, bb, c);
+        else return k < 0 ? med9_pre(b.l, a, bb, c) : med9_pre(b.v[k], a, bb, c);
     };
     // In rows 2 .. R - 3 nothing is read from `in`; the other rows wait for
     // their `in` loads inside their own branch (vmcnt(0), expcnt and lgkmcnt
     // left free: 0x0F70), so that the plain rows' stores do not wait for the
     // next row's statistics still in flight.
-    auto row = [&](int r, const uint64_t (&pr)[K], uint64_t prl) __attribute__((always_inline)) {
+    auto row = [&](int r, const MedBuf<K, RAW>& pb) __attribute__((always_inline)) {
         const int base = r * C;
         const bool plain = r >= 2 && r <= R - 3;
         unsigned v[K];
@@ -530,7 +571,7 @@ __device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t*
         for (int k = 0; k < K; k++) {
             if (!EXACT && k >= kact) break;
             const int c = col(k);
-            v[k] = med9_pre(pr[k], prev[c - 1], prev[c], prev[c + 1]);
+            v[k] = med(pb, k, prev[c - 1], prev[c], prev[c + 1]);
         }
         if (!plain) {
             // out(1, 0) = 0; from index qhi + 1 on the input is copied
@@ -555,7 +596,7 @@ __device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t*
         if (tid == 0) {
             // column C - 1 reads column 0 of this row, v[0] of this thread
             const int q = base + C - 1;
-            unsigned w = med9_pre(prl, prev[C - 2], prev[C - 1], v[0]);
+            unsigned w = med(pb, -1, prev[C - 2], prev[C - 1], v[0]);
             if (!plain) {
                 if (q > qhi) w = G(in)[q];
                 __builtin_amdgcn_s_waitcnt(0x0F70);
@@ -568,27 +609,32 @@ __device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t*
         uint16_t* t = prev; prev = cur; cur = t;
     };
     if constexpr (PF) {
-        uint64_t bufA[K], bufB[K], lastA, lastB;
-        fetch(1, bufA, lastA);
+        MedBuf<K, RAW> bufA, bufB;
+        fetch(1, bufA);
         __syncthreads();
         int r = 1;
         for (; r + 1 < R; r += 2) {
-            fetch(r + 1, bufB, lastB);
-            row(r, bufA, lastA);
-            fetch(min(r + 2, R - 1), bufA, lastA);
-            row(r + 1, bufB, lastB);
+            fetch(r + 1, bufB);
+            row(r, bufA);
+            fetch(min(r + 2, R - 1), bufA);
+            row(r + 1, bufB);
         }
-        if (r < R) row(r, bufA, lastA);
+        if (r < R) row(r, bufA);
     } else {
-        uint64_t buf[K], last;
+        MedBuf<K, RAW> buf;
         __syncthreads();
         for (int r = 1; r < R; r++) {
-            fetch(r, buf, last);
-            row(r, buf, last);
+            fetch(r, buf);
+            row(r, buf);
         }
     }
 }
 
+// MODE 0: frames up to kMedRawK * 1024 wide, sorting the six fixed values
+// itself; MODE 1: the same frames from the prepass; MODE 2: the wider frames,
+// from the prepass.  Each form has its own register budget; a kernel's blocks
+// of another form return at once.
+template <int MODE>
 __global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restrict__ rigs,
                                                         uint64_t* const* __restrict__ scratch)
 {
@@ -600,13 +646,19 @@ __global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restric
     const uint16_t* in = g.wta[m];
     uint16_t* out = g.med[m];
     const uint64_t* pre = scratch[ri] + (size_t)m * R * C;
-    switch ((C + 1023) >> 10) {
-    case 1: median_serial_body<1, true, true>(R, C, in, out, pre, lds16); break;
-    case 2: median_serial_body<2, true, true>(R, C, in, out, pre, lds16); break;
-    case 3: median_serial_body<3, true, true>(R, C, in, out, pre, lds16); break;
-    case 4: median_serial_body<4, true, true>(R, C, in, out, pre, lds16); break;
-    case 5: case 6: case 7: case 8: median_serial_body<8, false, true>(R, C, in, out, pre, lds16); break;
-    default: median_serial_body<kMedMaxK, false, false>(R, C, in, out, pre, lds16); break;
+    const int k = (C + 1023) >> 10;
+    if ((MODE == 2) != (k > kMedRawK)) return;
+    constexpr bool RAW = MODE == 0;
+    if constexpr (MODE < 2) {
+        switch (k) {
+        case 1: median_serial_body<1, true, true, RAW>(R, C, in, out, pre, lds16); break;
+        case 2: median_serial_body<2, true, true, RAW>(R, C, in, out, pre, lds16); break;
+        case 3: median_serial_body<3, true, true, RAW>(R, C, in, out, pre, lds16); break;
+        default: median_serial_body<4, true, true, RAW>(R, C, in, out, pre, lds16); break;
+        }
+    } else {
+        if (k <= 8) median_serial_body<8, false, true, false>(R, C, in, out, pre, lds16);
+        else median_serial_body<kMedMaxK, false, false, false>(R, C, in, out, pre, lds16);
     }
 }
 
@@ -2070,15 +2122,29 @@ void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
     TE(kKWta);
 }
 
+static size_t g_lds_per_block = 0;   // set by kernels_setup
+static int g_num_cu = 0;
+
 void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
 {
-    TB(kKMedianPrep);
-    hipLaunchKernelGGL(k_median_prep, dim3(ls.blocks[kStageMedianPrep]), dim3(256), 0, st, ls.rigs, ls.nrig,
-                       ls.bs[kStageMedianPrep], ls.scratch);
-    TE(kKMedianPrep);
-    size_t lds = (size_t)2 * (ls.maxCols + 1) * sizeof(uint16_t);
+    // With enough (rig, map) blocks to spread over the chip the serial kernel
+    // sorts each position's six fixed values itself (no prepass pass over
+    // memory); with few, each block's row chain is the critical path and the
+    // parallel prepass takes the sorting off it.  Frames wider than the raw
+    // form takes always use the prepass.
+    const size_t lds = (size_t)2 * (ls.maxCols + 1) * sizeof(uint16_t);
+    const bool raw = ls.nrig * 8 >= std::max(g_num_cu / 4, 1);
+    const bool wide = ls.maxCols > 1024 * kMedRawK;
+    if (wide || !raw) {
+        TB(kKMedianPrep);
+        hipLaunchKernelGGL(k_median_prep, dim3(ls.blocks[kStageMedianPrep]), dim3(256), 0, st, ls.rigs, ls.nrig,
+                           ls.bs[kStageMedianPrep], ls.scratch);
+        TE(kKMedianPrep);
+    }
     TB(kKMedianSerial);
-    hipLaunchKernelGGL(k_median_serial, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
+    if (wide) hipLaunchKernelGGL(k_median_serial<2>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
+    if (raw) hipLaunchKernelGGL(k_median_serial<0>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
+    else hipLaunchKernelGGL(k_median_serial<1>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
     TE(kKMedianSerial);
 }
 
@@ -2101,8 +2167,6 @@ int msum_blocks(const RigDesc& g, int side)
 }
 
 constexpr int kMsumTileMaxJ = 5;
-static size_t g_lds_per_block = 0;   // set by kernels_setup
-static int g_num_cu = 0;
 
 int kernels_setup(int device)
 {

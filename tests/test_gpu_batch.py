@@ -41,6 +41,29 @@ def test_batched_mixed_sizes_match_oracle(gpu_ctx, oracle_mod):
         assert float(np.abs(d["outs"]["subpix"].cpu().numpy() - ref["subpix"]).max()) <= 1e-3
 
 
+def test_batched_eight_rigs_medians_match_oracle(gpu_ctx, oracle_mod):
+    """Eight rigs give 64 (rig, map) blocks: enough for the serial median to
+    sort each position's six fixed values itself (one rig takes the prepass
+    form, which the stage tests cover).  Every median map, then the outputs."""
+    import torch
+    specs = MIX + [(64, 40, 16, 57), (48, 48, 24, 58)]
+    rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in specs]
+    dev = [_device_rig(gpu_ctx, r, torch) for r in rigs]
+    print("[batch] 8 rigs in one plan, medians", flush=True)
+    gpu_ctx.run_device([d["struct"] for d in dev], sister_amd.MODE_MULTIVIEW)
+    gpu_ctx.sync()
+    for k, ((W, H, D, _), r, d) in enumerate(zip(specs, rigs, dev)):
+        ref = oracle_mod.compute(r["views"], D, modes=1, debug=True)
+        Wp, Hp = W + 2 * D, H + 2 * D
+        shapes = [(Hp, Wp), (Wp, Hp), (Hp, Wp), (Wp, Hp)]
+        for F in range(4):
+            for side, nm in ((0, "med_l"), (1, "med_r")):
+                g = gpu_ctx.fetch(k, "med", 2 * F + side, shapes[F], np.uint16)
+                assert np.array_equal(g, ref[nm][F].astype(np.uint16)), (k, F, side)
+        assert np.array_equal(d["outs"]["multiview"].cpu().numpy(), ref["multiview"])
+        assert np.array_equal(d["outs"]["disp"].cpu().numpy(), ref["disp"])
+
+
 def test_graph_replays_match_eager(gpu_ctx):
     import torch
     rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in MIX[:4]]

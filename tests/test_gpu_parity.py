@@ -133,3 +133,14 @@ def test_plain_640x480_d64(gpu_ctx, oracle_mod):
     for k in ("multiview", "horizontal", "vertical", "disp"):
         assert np.array_equal(got[k], ref[k]), (k, int((got[k] != ref[k]).sum()))
     assert float(np.abs(got["subpix"] - ref["subpix"]).max()) <= 1e-3
+
+
+# frames wider than 4096 pixels take the serial median's prepass form (8 slots
+# per thread up to 8192, 16 beyond); the others sort the six fixed values in
+# the serial kernel.  W' = 4136 and 8232 wide frames 0/180, H' = 4136 tall
+# (so 4136-wide frames 90/270).
+@pytest.mark.parametrize("W,H,D,seed", [(4104, 16, 16, 21), (16, 4104, 16, 22), (8200, 12, 8, 23)])
+def test_stages_bit_exact_wide_frames(gpu_ctx, oracle_mod, W, H, D, seed):
+    print(f"[parity wide] W={W} H={H} D={D} seed={seed}", flush=True)
+    bad = _compare_stages(gpu_ctx, oracle_mod, W, H, D, seed)
+    assert not bad, bad
