@@ -631,9 +631,10 @@ __device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t*
 }
 
 // MODE 0: frames up to kMedRawK * 1024 wide, sorting the six fixed values
-// itself; MODE 1: the same frames from the prepass; MODE 2: the wider frames,
-// from the prepass.  Each form has its own register budget; a kernel's blocks
-// of another form return at once.
+// itself (its blocks of wider frames return at once); MODE 1: the same frames
+// from the prepass; MODE 2: every frame from the prepass, for plans with
+// frames wider than that (all maps at once: a rig's wide and narrow frames
+// take the same time, so they run side by side).
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restrict__ rigs,
                                                         uint64_t* const* __restrict__ scratch)
@@ -647,16 +648,16 @@ __global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restric
     uint16_t* out = g.med[m];
     const uint64_t* pre = scratch[ri] + (size_t)m * R * C;
     const int k = (C + 1023) >> 10;
-    if ((MODE == 2) != (k > kMedRawK)) return;
+    if (MODE < 2 && k > kMedRawK) return;
     constexpr bool RAW = MODE == 0;
-    if constexpr (MODE < 2) {
+    if (k <= kMedRawK) {
         switch (k) {
         case 1: median_serial_body<1, true, true, RAW>(R, C, in, out, pre, lds16); break;
         case 2: median_serial_body<2, true, true, RAW>(R, C, in, out, pre, lds16); break;
         case 3: median_serial_body<3, true, true, RAW>(R, C, in, out, pre, lds16); break;
         default: median_serial_body<4, true, true, RAW>(R, C, in, out, pre, lds16); break;
         }
-    } else {
+    } else if constexpr (MODE == 2) {
         if (k <= 8) median_serial_body<8, false, true, false>(R, C, in, out, pre, lds16);
         else median_serial_body<kMedMaxK, false, false, false>(R, C, in, out, pre, lds16);
     }
@@ -2130,20 +2131,20 @@ void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
     // With enough (rig, map) blocks to spread over the chip the serial kernel
     // sorts each position's six fixed values itself (no prepass pass over
     // memory); with few, each block's row chain is the critical path and the
-    // parallel prepass takes the sorting off it.  Frames wider than the raw
-    // form takes always use the prepass.
+    // parallel prepass takes the sorting off it.  Plans with frames wider
+    // than the raw form takes run every map from the prepass in one launch.
     const size_t lds = (size_t)2 * (ls.maxCols + 1) * sizeof(uint16_t);
-    const bool raw = ls.nrig * 8 >= std::max(g_num_cu / 4, 1);
     const bool wide = ls.maxCols > 1024 * kMedRawK;
-    if (wide || !raw) {
+    const bool raw = !wide && ls.nrig * 8 >= std::max(g_num_cu / 4, 1);
+    if (!raw) {
         TB(kKMedianPrep);
         hipLaunchKernelGGL(k_median_prep, dim3(ls.blocks[kStageMedianPrep]), dim3(256), 0, st, ls.rigs, ls.nrig,
                            ls.bs[kStageMedianPrep], ls.scratch);
         TE(kKMedianPrep);
     }
     TB(kKMedianSerial);
-    if (wide) hipLaunchKernelGGL(k_median_serial<2>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
     if (raw) hipLaunchKernelGGL(k_median_serial<0>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
+    else if (wide) hipLaunchKernelGGL(k_median_serial<2>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
     else hipLaunchKernelGGL(k_median_serial<1>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
     TE(kKMedianSerial);
 }
