@@ -668,28 +668,62 @@ __global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restric
 // (r,c) keeps L when 0 <= L <= c and |L - R[c-L]| <= 5; mask = kept && L > 0
 // && c >= D, written at the padded original coordinates (hpp:216,235-236,250-251).
 // ---------------------------------------------------------------------------
+// One block per 32x32 tile of frame pixels (4 per thread), read along frame
+// rows.  Frames 0/180 store along image rows directly; frames 90/270, whose
+// frame rows are image columns, pass the tile through LDS so that their
+// stores also run along image rows.
+constexpr int kMT32 = 32;
+
 __global__ __launch_bounds__(256) void k_mask(const RigDesc* __restrict__ rigs, int nrig,
                                               const int* __restrict__ bs, int fmask)
 {
+    __shared__ uint8_t tile[kMT32][kMT32 + 1];
     const int ri = find_rig(bs, nrig, blockIdx.x);
     const RigDesc& g = rigs[ri];
-    const long npx = (long)g.Wp * g.Hp;
-    const long perF = (npx + 255) / 256;
-    const long lb = blockIdx.x - bs[ri];
-    const int F = (int)(lb / perF);
-    const long q = (lb % perF) * 256 + threadIdx.x;
-    if (q >= npx || !(fmask & (1 << F))) return;
-    const int C = frame_cols(g, F);
-    const int r = (int)(q / C), c = (int)(q % C);
-    const int L = g.med[2 * F][q];
-    unsigned ok = 0;
-    if (c >= g.D && L >= 1 && L <= c) {
-        int Rv = g.med[2 * F + 1][q - L];
-        ok = abs(L - Rv) <= kLrThreshold;
+    const int T = ((g.Wp + kMT32 - 1) / kMT32) * ((g.Hp + kMT32 - 1) / kMT32);
+    const int lb = blockIdx.x - bs[ri];
+    const int F = lb / T, t = lb % T;
+    if (!(fmask & (1 << F))) return;
+    const int R = frame_rows(g, F), C = frame_cols(g, F);
+    const int tilesC = (C + kMT32 - 1) / kMT32;
+    const int r0 = (t / tilesC) * kMT32, c0 = (t % tilesC) * kMT32;
+    const int tc = threadIdx.x % kMT32, tr = threadIdx.x / kMT32;
+    const uint16_t* ml = g.med[2 * F];
+    const uint16_t* mr = g.med[2 * F + 1];
+    uint8_t* const mk = g.mask[F];
+    const bool odd = F & 1;
+#pragma unroll
+    for (int k = 0; k < kMT32 / 8; k++) {
+        const int r = r0 + tr + 8 * k, c = c0 + tc;
+        if (r >= R || c >= C) continue;
+        const size_t q = (size_t)r * C + c;
+        const int L = G(ml)[q];
+        unsigned ok = 0;
+        if (c >= g.D && L >= 1 && L <= c) {
+            const int Rv = G(mr)[q - L];
+            ok = abs(L - Rv) <= kLrThreshold;
+        }
+        if (odd) {
+            tile[tr + 8 * k][tc] = (uint8_t)ok;
+        } else {
+            int y, x;
+            frame_to_orig(g, F, r, c, y, x);
+            G(mk)[(size_t)y * g.Wp + x] = (uint8_t)ok;
+        }
     }
-    int y, x;
-    frame_to_orig(g, F, r, c, y, x);
-    g.mask[F][(size_t)y * g.Wp + x] = (uint8_t)ok;
+    if (!odd) return;
+    __syncthreads();
+    // frame (r, c) -> image (y, x): x = W'-1-r for both frames, y = H'-1-c
+    // (90) or c (270); a thread takes frame row r0 + tc (image column) and
+    // columns c0 + tr + 8 k (image rows), so lanes store along image rows
+#pragma unroll
+    for (int k = 0; k < kMT32 / 8; k++) {
+        const int r = r0 + tc, c = c0 + tr + 8 * k;
+        if (r >= R || c >= C) continue;
+        int y, x;
+        frame_to_orig(g, F, r, c, y, x);
+        G(mk)[(size_t)y * g.Wp + x] = tile[tc][tr + 8 * k];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2433,7 +2467,7 @@ void stage_blocks(const RigDesc& g, int maxD, bool wta_rows, int out[kNumStages]
         out[kStageWtaV] = 0;
     }
     out[kStageMedianPrep] = (int)(8 * ((npx + 255) / 256));
-    out[kStageMask] = (int)(4 * ((npx + 255) / 256));
+    out[kStageMask] = 4 * ((g.Wp + kMT32 - 1) / kMT32) * ((g.Hp + kMT32 - 1) / kMT32);
 }
 
 }  // namespace sister
