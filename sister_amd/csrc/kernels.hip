@@ -302,16 +302,23 @@ size_t wta_row_lds(int C, int D)
     return ((size_t)C + (size_t)C + D - 1 + 256) * sizeof(uint64_t) + (size_t)C * sizeof(uint32_t);
 }
 
-template <bool EDGE>
+// LFIX: a segment with pixels c < D - 1, whose left candidates d > c do not
+// exist: their keys are kept out of the left minima (the same costs feed the
+// right accumulators of pixels c - d < 0, which are dropped).  RFIX: a segment
+// reaching past the row: its pixels c >= C add 128 to their popcounts (the
+// popcount's addend), so their keys exceed every real one in the right minima
+// at no cost per step; their left results are not stored.
+template <bool LFIX, bool RFIX>
 __device__ __forceinline__ void wta_row_segment(const uint64_t* ccS, const uint64_t* cnS, uint32_t* RP,
                                                 uint16_t* outL, int x0, int C, int D, int lane)
 {
     const int cb = x0 + 4 * lane;
     uint64_t cci[4], w[4];
-    uint32_t bL[4], bR[4];
+    uint32_t bL[4], bR[4], ba[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        cci[i] = (!EDGE || cb + i < C) ? ccS[cb + i] : 0;
+        cci[i] = (!RFIX || cb + i < C) ? ccS[cb + i] : 0;
+        ba[i] = (RFIX && cb + i >= C) ? 128u : 0u;
         w[i] = cnS[cb + i + D - 1];
         bL[i] = 0xFFFFFFFFu;
         bR[i] = 0xFFFFFFFFu;
@@ -320,10 +327,10 @@ __device__ __forceinline__ void wta_row_segment(const uint64_t* ccS, const uint6
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const uint64_t x = cci[i] ^ w[i];
-            const uint32_t pop = bcnt_add((uint32_t)(x >> 32), bcnt_add((uint32_t)x, 0u));
+            const uint32_t pop = bcnt_add((uint32_t)(x >> 32), bcnt_add((uint32_t)x, ba[i]));
             const uint32_t key = (pop << 9) | (uint32_t)d;
-            bL[i] = min(bL[i], (EDGE && d > cb + i) ? 0xFFFFFFFFu : key);
-            bR[i] = min(bR[i], (EDGE && cb + i >= C) ? 0xFFFFFFFFu : key);
+            bL[i] = min(bL[i], (LFIX && d > cb + i) ? 0xFFFFFFFFu : key);
+            bR[i] = min(bR[i], key);
         }
     };
     auto shift = [&](int d) __attribute__((always_inline)) {
@@ -331,7 +338,9 @@ __device__ __forceinline__ void wta_row_segment(const uint64_t* ccS, const uint6
         const uint32_t out = bR[3];
         const uint32_t in = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)out, 0x138, 0xF, 0xF, false);
         const int pl = x0 + 255 - d;          // pixel of lane 63's leaving accumulator
-        if (lane == 63 && (!EDGE || pl < C)) atomicMin(&RP[pl], out);   // interior: x0 + 256 <= C
+        // interior segments: x0 + 256 <= C and x0 >= D - 1, so 0 <= pl < C; at
+        // the row's start pl < 0 once d > x0 + 255 (D > 256)
+        if (lane == 63 && (!LFIX || pl >= 0) && (!RFIX || pl < C)) atomicMin(&RP[pl], out);
         bR[3] = bR[2]; bR[2] = bR[1]; bR[1] = bR[0]; bR[0] = in;
         w[3] = w[2]; w[2] = w[1]; w[1] = w[0];
         w[0] = cnS[cb + D - 2 - d];           // CN[cb - (d + 1)]
@@ -352,7 +361,7 @@ __device__ __forceinline__ void wta_row_segment(const uint64_t* ccS, const uint6
     step(d + 3);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        if (!EDGE || cb + i < C) G(outL)[cb + i] = (uint16_t)(bL[i] & 511u);
+        if (!RFIX || cb + i < C) G(outL)[cb + i] = (uint16_t)(bL[i] & 511u);
         const int pp = cb + i - (D - 1);      // pixel of R[i] after the last step
         if (pp >= 0 && pp < C) atomicMin(&RP[pp], bR[i]);
     }
@@ -394,8 +403,10 @@ __global__ __launch_bounds__(64 * kWtaRowMaxWaves) void k_wta_row(const RigDesc*
         // segments at the row's ends mask the candidates that do not exist
         // (fewer than D at the start, pixels past the row at the end)
         const bool lfix = x0 < D - 1, rfix = x0 + 256 > C;
-        if (lfix || rfix) wta_row_segment<true>(ccS, cnS, RP, outL, x0, C, D, lane);
-        else wta_row_segment<false>(ccS, cnS, RP, outL, x0, C, D, lane);
+        if (lfix && rfix) wta_row_segment<true, true>(ccS, cnS, RP, outL, x0, C, D, lane);
+        else if (lfix) wta_row_segment<true, false>(ccS, cnS, RP, outL, x0, C, D, lane);
+        else if (rfix) wta_row_segment<false, true>(ccS, cnS, RP, outL, x0, C, D, lane);
+        else wta_row_segment<false, false>(ccS, cnS, RP, outL, x0, C, D, lane);
     }
     __syncthreads();
     uint16_t* const outR = g.wta[2 * F + 1] + (size_t)r * C;
