@@ -752,6 +752,18 @@ __global__ __launch_bounds__(256) void k_mask(const RigDesc* __restrict__ rigs, 
 // LDS reads at immediate offsets and coalesced 64-byte stores per group, with
 // no per-disparity branches.  Cell stride Dp, zero past D.
 // ---------------------------------------------------------------------------
+// compile-time loop: f(integral_constant<int, I>) for I in [B, E), so register
+// arrays indexed by I are never indexed at run time
+template <int I> struct IC { static constexpr int value = I; };
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(IC<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 constexpr int kRun = 256;   // pixels per block
 constexpr int kWinPad = 8;  // window entries before each partner window
 
@@ -885,9 +897,16 @@ __global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ ri
 // ---------------------------------------------------------------------------
 constexpr int kMT = 16;
 
-size_t msum_tile_lds(int D)
+// window line stride (entries) for J groups of 64 disparities: holds
+// kWinPad + kMT + D - 1 <= 64 J + 23 entries; a compile-time constant, so a
+// pixel's reads sit at immediate offsets from per-frame lane addresses
+template <int J>
+constexpr int msum_lstr() { return 64 * J + 24; }
+
+size_t msum_tile_lds(int Dp)
 {
-    return ((size_t)4 * kMT * (kMT + D - 1 + kWinPad) + 4 * kMT * kMT) * 8 + kMT * kMT;
+    const int J = (Dp + 63) / 64;
+    return ((size_t)4 * kMT * (64 * J + 24) + 4 * kMT * kMT) * 8 + kMT * kMT;
 }
 
 template <int J>
@@ -899,7 +918,8 @@ __global__ __launch_bounds__(1024) void k_msum_tile(const RigDesc* __restrict__ 
     constexpr int NK = J + 1;
     extern __shared__ uint64_t lds64[];
     const int D = rigs[0].D, Dp = rigs[0].Dp;   // one disparity count per launch
-    const int SPAN = kMT + D - 1, LSTR = SPAN + kWinPad;
+    const int SPAN = kMT + D - 1;
+    constexpr int LSTR = msum_lstr<J>();
     auto win = [&](int F, int l) { return lds64 + (F * kMT + l) * LSTR + kWinPad; };
     uint64_t* CEN = lds64 + 4 * kMT * LSTR;     // [frame][pixel]
     uint8_t* MB = (uint8_t*)(CEN + 4 * kMT * kMT);
@@ -987,31 +1007,55 @@ __global__ __launch_bounds__(1024) void k_msum_tile(const RigDesc* __restrict__ 
     const int qlast = min(q, Dp - 1 - 64 * (J - 1));
     const bool store_last = q + 64 * (J - 1) < Dp;
     const bool live_last = q + 64 * (J - 1) < D;
-    // wave w walks tile column w
+    // wave w walks tile column w, its 16 pixels fully unrolled: every read
+    // sits at a compile-time offset from one lane address per frame, so no
+    // address arithmetic runs per pixel.  Frame line and window index of
+    // pixel (ty, tx) at disparity d:
+    //   F0: line ty, index tx + D - 1 - d;         F2: line ty, index (kMT-1-tx) + D - 1 - d
+    //   F1: line tx, index (kMT-1-ty) + D - 1 - d; F3: line tx, index ty + D - 1 - d
     auto sum_tile = [&](const Geo& gq) __attribute__((always_inline)) {
         const RigDesc& g = rigs[gq.ri];
         const int Wp = g.Wp, Hp = g.Hp;
-        uint8_t* const msum = g.msum;
         const int tx = wave, x = gq.x0 + tx;
         if (x >= Wp) return;
         const uint32_t mcol = lane < kMT ? MB[lane * kMT + tx] : 0u;   // lane ty: mask bits of (ty, tx)
-        for (int ty = 0; ty < kMT; ty++) {
-            const int y = gq.y0 + ty;
-            if (y >= Hp) break;
-            const int i = ty * kMT + tx;
+        // group j of lane q reads entry base - q - 64 j: groups below J - 1 from
+        // pa + 64 (J - 1 - j), the last group (lanes past Dp read as qlast) from pl
+        const int lo = D - 1 - 64 * (J - 1);
+        const uint64_t* pa[4] = {win(0, 0) + tx + lo - q, win(1, tx) + lo - q,
+                                 win(2, 0) + (kMT - 1 - tx) + lo - q, win(3, tx) + lo - q};
+        const uint64_t* pl[4] = {win(0, 0) + tx + lo - qlast, win(1, tx) + lo - qlast,
+                                 win(2, 0) + (kMT - 1 - tx) + lo - qlast, win(3, tx) + lo - qlast};
+        const uint64_t* pc0 = CEN + tx;
+        gptr<uint8_t> o = G(g.msum) + ((size_t)gq.y0 * Wp + x) * Dp;
+        const size_t rstride = (size_t)Wp * Dp;
+        static_for<0, kMT>([&](auto tyc) __attribute__((always_inline)) {
+            constexpr int ty = decltype(tyc)::value;
+            if (gq.y0 + ty >= Hp) return;
             unsigned acc[J];
 #pragma unroll
             for (int j = 0; j < J; j++) acc[j] = 0;
             const int mb = __builtin_amdgcn_readlane(mcol, ty);   // one pixel per wave
-            if (mb & 1) msum_add_frame<J>(win(0, ty), CEN[i], tx + D - 1, q, qlast, acc);
-            if (mb & 4) msum_add_frame<J>(win(2, ty), CEN[2 * kMT * kMT + i], (kMT - 1 - tx) + D - 1, q, qlast, acc);
-            if (mb & 2) msum_add_frame<J>(win(1, tx), CEN[kMT * kMT + i], (kMT - 1 - ty) + D - 1, q, qlast, acc);
-            if (mb & 8) msum_add_frame<J>(win(3, tx), CEN[3 * kMT * kMT + i], ty + D - 1, q, qlast, acc);
-            gptr<uint8_t> o = G(msum) + ((size_t)y * Wp + x) * Dp;
+            static_for<0, 4>([&](auto kc) __attribute__((always_inline)) {
+                constexpr int F = (0x3120 >> (4 * decltype(kc)::value)) & 15;   // frames 0, 2, 1, 3
+                constexpr int off = F == 0 || F == 2 ? ty * LSTR : F == 1 ? kMT - 1 - ty : ty;
+                if (!(mb & (1 << F))) return;
+                uint64_t tv[J];
 #pragma unroll
-            for (int j = 0; j < J - 1; j++) o[q + 64 * j] = (uint8_t)acc[j];
-            if (store_last) o[q + 64 * (J - 1)] = live_last ? (uint8_t)acc[J - 1] : (uint8_t)0;
-        }
+                for (int j = 0; j < J - 1; j++) tv[j] = pa[F][off + 64 * (J - 1 - j)];
+                tv[J - 1] = pl[F][off];
+                const uint64_t cc = pc0[F * kMT * kMT + ty * kMT];
+#pragma unroll
+                for (int j = 0; j < J; j++) {
+                    const uint64_t xv = cc ^ tv[j];
+                    acc[j] = bcnt_add((uint32_t)(xv >> 32), bcnt_add((uint32_t)xv, acc[j]));
+                }
+            });
+            gptr<uint8_t> op = o + ty * rstride;
+#pragma unroll
+            for (int j = 0; j < J - 1; j++) op[q + 64 * j] = (uint8_t)acc[j];
+            if (store_last) op[q + 64 * (J - 1)] = live_last ? (uint8_t)acc[J - 1] : (uint8_t)0;
+        });
     };
     Geo cur = geo(t);
     load(cur);
@@ -1062,18 +1106,6 @@ __global__ __launch_bounds__(1024) void k_msum_tile(const RigDesc* __restrict__ 
 // ---------------------------------------------------------------------------
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 constexpr uint32_t kGuard2 = 0x3FFF3FFFu;
-
-// compile-time loop: f(integral_constant<int, I>) for I in [B, E), so register
-// arrays indexed by the chunk number are never indexed at run time
-template <int I> struct IC { static constexpr int value = I; };
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F&& f)
-{
-    if constexpr (B < E) {
-        f(IC<B>{});
-        static_for<B + 1, E>(f);
-    }
-}
 
 __device__ __forceinline__ us2 U2(uint32_t v) { return __builtin_bit_cast(us2, v); }
 __device__ __forceinline__ uint32_t W2(us2 v) { return __builtin_bit_cast(uint32_t, v); }
@@ -2263,12 +2295,12 @@ static bool msum_split_forced()
 
 static bool msum_tile_fits(const DGroup& G)
 {
-    return (G.Dp + 63) / 64 <= kMsumTileMaxJ && msum_tile_lds(G.D) <= g_lds_per_block;
+    return (G.Dp + 63) / 64 <= kMsumTileMaxJ && msum_tile_lds(G.Dp) <= g_lds_per_block;
 }
 
 static void launch_msum_tile(const DGroup& G, int fuse, hipStream_t st)
 {
-    const size_t lds = msum_tile_lds(G.D);
+    const size_t lds = msum_tile_lds(G.Dp);
     const int ntiles = G.blocks[kGMsumTile];
     const dim3 grid(std::min(ntiles, std::max(g_num_cu, 1))), block(1024);   // persistent, one per CU
     const int* bs = G.bs[kGMsumTile];
