@@ -1632,7 +1632,8 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 // Buffers: P lives in part[1..2] (u16 per cell, in row planes when FULL), the
 // rows family's two directions in part[0] and rowsb (= part[3] here; bytes,
 // added as words: each sum is at most 200), the segment states in vstate
-// ([parity][cols, diag, anti][W'][Dp/2 + 4] words: pairs, then minL').
+// ([parity][cols, diag, anti][W'][Dp + 16] bytes: the normalised (L - M)
+// bytes, then minL').
 // ---------------------------------------------------------------------------
 constexpr int kVGhost = 16;        // ghost columns per side = rows per launch
 constexpr int kVCols = 144;        // owned columns per block (16 blocks per 2304-wide rig)
@@ -1763,7 +1764,7 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         return Pd + ((size_t)y * Wp + x) * Dp + boff + 16 * q + 8 * h;
     };
 
-    // lane state <-> a column vector in memory (pairs, then minL' at word Dp/2)
+    // lane state <-> a column's segment state (bytes, then minL' at byte Dp)
     // mr: the cost bytes M(r) of the segment's last row r (the previous
     // segment's, when loading); padding pairs come back as guards
     auto st_load = [&](LaneState<CH>& st, const uint8_t* v, const uint4 (&mr)[CH]) __attribute__((always_inline)) {
