@@ -1321,7 +1321,9 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
     // two directions run as separate waves, the grid doubled and its second
     // half sweeping backward; a wave holding the top or bottom row's chain,
     // whose exact edge values add both directions, sweeps both in its
-    // first-half block.
+    // first-half block.  For the anti-diagonal family rsplit 1 / 2 runs only
+    // the forward / backward (final) sweep: the forward one needs M alone, so
+    // it can run beside the other families.
     extern __shared__ uint32_t lds_s[];   // final sweep: S pairs of the wave's chains
     constexpr int CPW = 64 / KL;          // chains per wave
     constexpr int NO = (FAM == 3) ? 4 : 1;
@@ -1383,6 +1385,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
     uint16_t* outm = g.out[mode];
 
     int dlo = 0, dhi = 1;
+    if (FAM == 3 && rsplit) dlo = dhi = rsplit - 1;
     if (dsel >= 0) {
         const bool edge_wave = cw0 == 0 || cw1 >= Hp - 1;
         if (edge_wave) {
@@ -2439,8 +2442,9 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         }
         return;
     }
-    // rows on st, columns and diagonals on the side streams, joined before
-    // the anti-diagonal family
+    // rows on st, columns and diagonals (then the anti-diagonal family's
+    // forward sweep) on the side streams, joined before the anti-diagonal
+    // family's final sweep
     hipStream_t sc = st, sd = st;
     if (fs) {
         (void)hipEventRecord(fs->fork, st);
@@ -2461,6 +2465,11 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         TB_S(kKDiag, sd);
         SISTER_FAM(G, 2, dim3(G.blocks[2]), dim3(64), 0, sd, G.rigs, G.nrig, G.bs[2], mode, keep_sum, 0);
         TE_S(kKDiag, sd);
+        // the anti-diagonal family's forward sweep needs M alone: it follows
+        // the diagonals on their stream, beside the rows
+        TB_S(kKAnti, sd);
+        SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), 0, sd, G.rigs, G.nrig, G.bs[3], mode, keep_sum, 1);
+        TE_S(kKAnti, sd);
     }
     if (fs) {
         (void)hipEventRecord(fs->join[0], sc);
@@ -2481,7 +2490,7 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         }
         const size_t lds = (size_t)64 * G.CH[3] * 32;
         TB_S(kKAnti, st);
-        SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[3], mode, keep_sum, 0);
+        SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[3], mode, keep_sum, 2);
         TE_S(kKAnti, st);
     }
 }
