@@ -41,6 +41,29 @@ def test_batched_mixed_sizes_match_oracle(gpu_ctx, oracle_mod):
         assert float(np.abs(d["outs"]["subpix"].cpu().numpy() - ref["subpix"]).max()) <= 1e-3
 
 
+def test_batched_fused_vertical_mixed_sizes_match_oracle(gpu_ctx, oracle_mod):
+    """The fused vertical sweep (forced on) over rigs of one D with different
+    widths and heights in one plan: per-rig block ranges, rigs that end
+    segments early (H' of 104, 136, 168 and 92 rows: 6 to 11 segments of 16),
+    plus a second D group beside them."""
+    import torch
+    specs = [(200, 40, 32, 61), (96, 72, 32, 62), (64, 104, 32, 63), (152, 28, 32, 64), (80, 60, 24, 65)]
+    rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in specs]
+    dev = [_device_rig(gpu_ctx, r, torch) for r in rigs]
+    print("[batch] 5 mixed rigs, fused vertical sweep", flush=True)
+    gpu_ctx.vsweep(1)
+    try:
+        gpu_ctx.run_device([d["struct"] for d in dev], sister_amd.MODE_MULTIVIEW)
+        gpu_ctx.sync()
+    finally:
+        gpu_ctx.vsweep(-1)
+    for r, d in zip(rigs, dev):
+        ref = oracle_mod.compute(r["views"], r["D"], modes=1)
+        assert np.array_equal(d["outs"]["multiview"].cpu().numpy(), ref["multiview"])
+        assert np.array_equal(d["outs"]["disp"].cpu().numpy(), ref["disp"])
+        assert float(np.abs(d["outs"]["subpix"].cpu().numpy() - ref["subpix"]).max()) <= 1e-3
+
+
 def test_batched_eight_rigs_medians_match_oracle(gpu_ctx, oracle_mod):
     """Eight rigs give 64 (rig, map) blocks: enough for the serial median to
     sort each position's six fixed values itself (one rig takes the prepass
