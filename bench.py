@@ -376,7 +376,7 @@ def main():
         run_single(args)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (rigs are independent, "
+        print(json.dumps({"error": "bench.py measures one GPU only; --gpus must be 1 (rigs are independent, "
                                    "a multi-GPU run would shard rigs across processes)"}))
         sys.exit(2)
     out = run_serve(args) if args.workload == "serve" else run(args)
