@@ -10,6 +10,8 @@
    SGM as eight independent path chains (the decomposition the GPU uses) with
    the reference's first-line, border and restart rules.
 4. A planted mistake in the chain restatement must be caught.
+5. The sub-pixel map (no reference counterpart) against the vertex of the
+   parabola through S at the winner and its neighbours.
 """
 import glob
 import hashlib
@@ -281,3 +283,33 @@ def test_pipeline_invariants(oracle_mod):
         col = [xs, Hp - 1 - ys, Wp - 1 - xs, ys]   # frame column of (y, x) per frame
         for F in range(4):
             assert np.all(col[F][r["mask"][F] == 1] >= D), "mask implies frame column >= D"
+
+
+def test_subpix_is_vertex_of_interpolating_parabola(oracle_mod):
+    """The sub-pixel map (absent in the reference; a parabola on S around the
+    winner) against the mathematics: where 0 < d* < D-1 and the three values
+    curve upward, the vertex of the quadratic that numpy.polyfit fits exactly
+    through (d*-1, S-), (d*, S0), (d*+1, S+); elsewhere d* itself.  A
+    minimum's vertex lies within half a pixel of it.  A swapped neighbour or
+    a dropped factor 2 moves the vertex off these."""
+    for W, H, D, seed in ((64, 48, 16, 41), (96, 64, 24, 42)):
+        rig = synth.rig_small(W, H, D, seed)
+        r = oracle_mod.compute(rig["views"], D, modes=1, debug=True)
+        S = r["sum"][0][D:D + H, D:D + W].astype(np.float64)       # cropped aggregate
+        d = r["disp"].astype(np.int64)
+        sp = r["subpix"].astype(np.float64)
+        assert np.array_equal(d, np.argmin(S, axis=2)), "winner = lowest-index argmin of S"
+        refined = 0
+        for y in range(H):
+            for x in range(W):
+                k = int(d[y, x])
+                want = float(k)
+                if 0 < k < D - 1:
+                    v = S[y, x, k - 1:k + 2]
+                    if v[0] - 2 * v[1] + v[2] > 0:
+                        a, b, _ = np.polyfit([k - 1, k, k + 1], v, 2)
+                        want = -b / (2 * a)
+                        refined += 1
+                assert abs(sp[y, x] - want) <= 1e-4, (y, x, k, sp[y, x], want)
+                assert abs(sp[y, x] - k) <= 0.5
+        assert refined > W * H // 2, refined
