@@ -144,3 +144,26 @@ def test_stages_bit_exact_wide_frames(gpu_ctx, oracle_mod, W, H, D, seed):
     print(f"[parity wide] W={W} H={H} D={D} seed={seed}", flush=True)
     bad = _compare_stages(gpu_ctx, oracle_mod, W, H, D, seed)
     assert not bad, bad
+
+
+# disparity counts beyond 192: wider SGM lane states, the tiled cost sum at 4
+# groups of 64 (D = 200, 256) and the two-launch cost sum beyond, the row WTA's
+# start-of-row segment when D > 256 (lane 63's leaving pixel before column 0),
+# and D = 512, the largest accepted.  Tiny visible sizes keep the oracle's
+# volumes in seconds (W' x H' x D up to 543 M cells at D = 512).
+@pytest.mark.parametrize("W,H,D,seed", [(16, 16, 200, 81), (16, 16, 256, 82), (16, 16, 264, 83),
+                                        (16, 12, 320, 84), (8, 8, 384, 85), (8, 4, 512, 86)])
+def test_large_disparity_counts_outputs(gpu_ctx, oracle_mod, W, H, D, seed):
+    print(f"[parity large D] W={W} H={H} D={D} seed={seed}", flush=True)
+    rig = synth.rig_small(W, H, D, seed)
+    ref = oracle_mod.compute(rig["views"], D, modes=1)
+    got = gpu_ctx.compute_rig(rig["views"], D, modes=1)
+    for k in ("multiview", "disp"):
+        assert np.array_equal(got[k], ref[k]), (k, int((got[k] != ref[k]).sum()))
+    assert float(np.abs(got["subpix"] - ref["subpix"]).max()) <= 1e-3
+
+
+def test_stages_bit_exact_d264(gpu_ctx, oracle_mod):
+    print("[parity stages] W=16 H=16 D=264 seed=87", flush=True)
+    bad = _compare_stages(gpu_ctx, oracle_mod, 16, 16, 264, 87)
+    assert not bad, bad
