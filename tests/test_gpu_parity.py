@@ -167,3 +167,50 @@ def test_stages_bit_exact_d264(gpu_ctx, oracle_mod):
     print("[parity stages] W=16 H=16 D=264 seed=87", flush=True)
     bad = _compare_stages(gpu_ctx, oracle_mod, 16, 16, 264, 87)
     assert not bad, bad
+
+
+# the smallest rigs accepted (W, H multiples of 4, D = 8): 20 x 20 padded frames,
+# smaller than one cost-sum tile, one mask tile or one WTA segment
+@pytest.mark.parametrize("W,H,D,seed", [(4, 4, 8, 88), (8, 4, 8, 89)])
+def test_smallest_rigs_bit_exact(gpu_ctx, oracle_mod, W, H, D, seed):
+    print(f"[parity smallest] W={W} H={H} D={D} seed={seed}", flush=True)
+    bad = _compare_stages(gpu_ctx, oracle_mod, W, H, D, seed)
+    assert not bad, bad
+
+
+def test_row_stride_host_and_device(gpu_ctx, oracle_mod):
+    """Views inside wider rows (stride > W), as the reference's Mats of a
+    larger image would be: the host form copies the rows out, the device
+    form reads them in place.  The padding bytes differ from the image."""
+    import ctypes as C
+
+    import torch
+
+    import sister_amd
+    W, H, D, pitch = 64, 48, 16, 84
+    rig = synth.rig_small(W, H, D, 90)
+    ref = oracle_mod.compute(rig["views"], D, modes=1)
+    print(f"[parity stride] W={W} H={H} D={D} stride={pitch}", flush=True)
+    bufs = []
+    for v in rig["views"]:
+        b = np.full((H, pitch), 201, np.uint8)
+        b[:, :W] = v
+        bufs.append(b)
+    r = sister_amd.RigStruct()
+    r.width, r.height, r.stride, r.disp_count = W, H, pitch, D
+    for i, b in enumerate(bufs):
+        r.view[i] = b.ctypes.data
+    mv = np.zeros((H, W), np.uint16)
+    dp = np.zeros((H, W), np.uint16)
+    r.out_multiview, r.out_disp = mv.ctypes.data, dp.ctypes.data
+    gpu_ctx._check(sister_amd.lib().sister_compute_rig(gpu_ctx._h, C.byref(r), sister_amd.MODE_MULTIVIEW))
+    assert np.array_equal(mv, ref["multiview"]) and np.array_equal(dp, ref["disp"])
+    dev_bufs = [torch.from_numpy(b).cuda() for b in bufs]
+    views = [t[:, :W] for t in dev_bufs]
+    assert views[0].stride(0) == pitch
+    outs = {"multiview": torch.zeros((H, W), dtype=torch.uint16, device="cuda"),
+            "disp": torch.zeros((H, W), dtype=torch.uint16, device="cuda")}
+    gpu_ctx.run_device([gpu_ctx.rig_struct(views, D, outs)], sister_amd.MODE_MULTIVIEW)
+    gpu_ctx.sync()
+    assert np.array_equal(outs["multiview"].cpu().numpy(), ref["multiview"])
+    assert np.array_equal(outs["disp"].cpu().numpy(), ref["disp"])
