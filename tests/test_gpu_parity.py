@@ -214,3 +214,18 @@ def test_row_stride_host_and_device(gpu_ctx, oracle_mod):
     gpu_ctx.sync()
     assert np.array_equal(outs["multiview"].cpu().numpy(), ref["multiview"])
     assert np.array_equal(outs["disp"].cpu().numpy(), ref["disp"])
+
+
+# subsets of the three modes (hpp:82,87: horizontal uses frames 0/180 only,
+# vertical 90/270 only); each output alone equals the oracle's
+@pytest.mark.parametrize("modes", [2, 4, 6, 3, 5])
+def test_mode_subsets(gpu_ctx, oracle_mod, modes):
+    W, H, D, seed = 96, 64, 24, 92
+    print(f"[parity modes] modes={modes} W={W} H={H} D={D}", flush=True)
+    rig = synth.rig_small(W, H, D, seed)
+    ref = oracle_mod.compute(rig["views"], D, modes=modes)
+    got = gpu_ctx.compute_rig(rig["views"], D, modes=modes)
+    names = [n for b, n in ((1, "multiview"), (2, "horizontal"), (4, "vertical")) if modes & b]
+    assert sorted(k for k in got if k in ("multiview", "horizontal", "vertical")) == sorted(names)
+    for k in names + (["disp"] if modes & 1 else []):
+        assert np.array_equal(got[k], ref[k]), (k, int((got[k] != ref[k]).sum()))
