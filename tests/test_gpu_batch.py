@@ -254,3 +254,33 @@ def test_full_1080_d192_bit_exact_vs_reference_kernels(gpu_ctx, oracle_mod):
     assert got["disp"].max() <= rig["D"] - 1
     assert np.array_equal(got["multiview"].astype(np.uint32), np.minimum(255 * got["disp"].astype(np.uint32), 65535))
     assert np.all(np.abs(got["subpix"] - got["disp"]) <= 0.5 + 1e-6)
+
+
+def test_graph_replays_fused_vertical_match_oracle(gpu_ctx, oracle_mod):
+    """A captured batch on the fused vertical sweep (forced on at capture), in
+    two graph parts, replayed three times with unrelated eager work between
+    replays: every replay equals the oracle."""
+    import torch
+    specs = [(200, 40, 32, 71), (96, 72, 32, 72), (64, 104, 32, 73), (152, 28, 32, 74)]
+    rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in specs]
+    refs = [oracle_mod.compute(r["views"], r["D"], modes=1) for r in rigs]
+    dev = [_device_rig(gpu_ctx, r, torch) for r in rigs]
+    other = _device_rig(gpu_ctx, synth.rig_small(64, 48, 16, 75), torch)
+    print("[graph] capture 4 rigs on the fused sweep, replay 3x", flush=True)
+    gpu_ctx.vsweep(1)
+    try:
+        gpu_ctx.graph_capture([d["struct"] for d in dev], sister_amd.MODE_MULTIVIEW)
+    finally:
+        gpu_ctx.vsweep(-1)
+    for rep in range(3):
+        for d in dev:
+            d["outs"]["disp"].zero_()
+            d["outs"]["multiview"].zero_()
+        torch.cuda.synchronize()
+        gpu_ctx.run_device([other["struct"]], 1)
+        gpu_ctx.sync()
+        gpu_ctx.graph_launch()
+        gpu_ctx.sync()
+        for ref, d in zip(refs, dev):
+            assert np.array_equal(d["outs"]["disp"].cpu().numpy(), ref["disp"]), rep
+            assert np.array_equal(d["outs"]["multiview"].cpu().numpy(), ref["multiview"]), rep
