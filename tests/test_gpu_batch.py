@@ -284,3 +284,21 @@ def test_graph_replays_fused_vertical_match_oracle(gpu_ctx, oracle_mod):
         for ref, d in zip(refs, dev):
             assert np.array_equal(d["outs"]["disp"].cpu().numpy(), ref["disp"]), rep
             assert np.array_equal(d["outs"]["multiview"].cpu().numpy(), ref["multiview"]), rep
+
+
+def test_full_1080_d192_fused_vertical_equals_separate(gpu_ctx):
+    """At full size the fused vertical sweep (forced on for the one rig: 16
+    blocks of 144 columns, 92 segments of 16 rows a pass) gives what the
+    separate families give; those equal the reference kernels at this size
+    (the test above).  Every output, S-derived sub-pixel map included."""
+    rig = synth.rig_plain_1080()
+    print("[full] 1920x1080 D=192 seed=2 fused vs separate vertical families", flush=True)
+    gpu_ctx.vsweep(0)
+    try:
+        sep = gpu_ctx.compute_rig(rig["views"], rig["D"], modes=7)
+        gpu_ctx.vsweep(1)
+        fused = gpu_ctx.compute_rig(rig["views"], rig["D"], modes=7)
+    finally:
+        gpu_ctx.vsweep(-1)
+    for k in ("multiview", "horizontal", "vertical", "disp", "subpix"):
+        assert np.array_equal(fused[k], sep[k]), (k, int((fused[k] != sep[k]).sum()))
