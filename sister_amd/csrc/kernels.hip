@@ -2248,7 +2248,9 @@ int msum_blocks(const RigDesc& g, int side)
     return side == 0 ? g.Hp * ((g.Wp + kRun - 1) / kRun) : g.Wp * ((g.Hp + kRun - 1) / kRun);
 }
 
-constexpr int kMsumTileMaxJ = 5;
+// at 5 groups (D > 256) the tile's windows need 184,576 bytes of LDS, over
+// the 160 KB a block can hold: those disparity counts take the two-launch form
+constexpr int kMsumTileMaxJ = 4;
 
 int kernels_setup(int device)
 {
@@ -2257,8 +2259,7 @@ int kernels_setup(int device)
     const int lim = v;
     if (hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return -1;
     const void* fns[kMsumTileMaxJ] = {(const void*)k_msum_tile<1>, (const void*)k_msum_tile<2>,
-                                      (const void*)k_msum_tile<3>, (const void*)k_msum_tile<4>,
-                                      (const void*)k_msum_tile<5>};
+                                      (const void*)k_msum_tile<3>, (const void*)k_msum_tile<4>};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
             (void)hipGetLastError();
@@ -2268,15 +2269,14 @@ int kernels_setup(int device)
     g_lds_per_block = (size_t)lim;
 #define SISTER_VS_FNS(K, F)                                                                         \
     (const void*)k_vsweep<1, 0, K, F>, (const void*)k_vsweep<2, 0, K, F>, (const void*)k_vsweep<3, 0, K, F>, \
-        (const void*)k_vsweep<4, 0, K, F>, (const void*)k_vsweep<1, 1, K, F>, (const void*)k_vsweep<2, 1, K, F>, \
-        (const void*)k_vsweep<3, 1, K, F>, (const void*)k_vsweep<4, 1, K, F>
+        (const void*)k_vsweep<1, 1, K, F>, (const void*)k_vsweep<2, 1, K, F>, (const void*)k_vsweep<3, 1, K, F>
     for (const void* f : {(const void*)k_wta_row<false>, (const void*)k_wta_row<true>})
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
             (void)hipGetLastError();
             g_lds_per_block = 0;
             return 0;
         }
-    const void* vfns[24] = {SISTER_VS_FNS(false, false), SISTER_VS_FNS(false, true), SISTER_VS_FNS(true, false)};
+    const void* vfns[18] = {SISTER_VS_FNS(false, false), SISTER_VS_FNS(false, true), SISTER_VS_FNS(true, false)};
 #undef SISTER_VS_FNS
     for (const void* f : vfns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
@@ -2312,8 +2312,7 @@ static void launch_msum_tile(const DGroup& G, int fuse, hipStream_t st)
     case 1: hipLaunchKernelGGL(k_msum_tile<1>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
     case 2: hipLaunchKernelGGL(k_msum_tile<2>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
     case 3: hipLaunchKernelGGL(k_msum_tile<3>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
-    case 4: hipLaunchKernelGGL(k_msum_tile<4>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
-    default: hipLaunchKernelGGL(k_msum_tile<5>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
+    default: hipLaunchKernelGGL(k_msum_tile<4>, grid, block, lds, st, G.rigs, G.nrig, bs, ntiles, fuse); break;
     }
 }
 
@@ -2390,7 +2389,7 @@ static void launch_vsweep(const DGroup& G, int mode, int keep_sum, hipStream_t s
     const dim3 grid(G.blocks[kGVSweep]), block(kVKL * nt);
     const int* bs = G.bs[kGVSweep];
     const int nseg = (G.maxHp + kVGhost - 1) / kVGhost;
-    const int ch = vs_ch(G.D);
+    const int ch = vs_ch(G.D);   // 1..3: vsweep_pick_cols gives no columns beyond 3 chunks
     const bool full = G.D == G.Dp && G.Dp == 64 * ch;
     for (int dir = 0; dir < 2; dir++)
         for (int seg = 0; seg < nseg; seg++) {
@@ -2410,15 +2409,13 @@ static void launch_vsweep(const DGroup& G, int mode, int keep_sum, hipStream_t s
                 switch (ch) {
                 case 1: SISTER_VS(1, 0); break;
                 case 2: SISTER_VS(2, 0); break;
-                case 3: SISTER_VS(3, 0); break;
-                default: SISTER_VS(4, 0); break;
+                default: SISTER_VS(3, 0); break;
                 }
             } else {
                 switch (ch) {
                 case 1: SISTER_VS(1, 1); break;
                 case 2: SISTER_VS(2, 1); break;
-                case 3: SISTER_VS(3, 1); break;
-                default: SISTER_VS(4, 1); break;
+                default: SISTER_VS(3, 1); break;
                 }
             }
 #undef SISTER_VS
