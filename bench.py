@@ -6,7 +6,9 @@ mode 0 (multiview).  Two forms are measured on the same synthetic rigs, with
 inputs and outputs resident in HBM before the timed region:
 
   plain    one rig per call of the library (sister_run_device with n = 1),
-  batched  B rigs in one plan, captured once as a hipGraph and replayed.
+  batched  B rigs in one plan (default 32: 125 GB of workspace, and as
+           much again for the eager plan kept beside the graph), captured
+           once as a hipGraph and replayed.
 
 `value` is the batched throughput in G disparity-evals/s (sum of W*H*D over
 the rigs of a step / step time); the plain figures and the ratio sit beside
@@ -56,7 +58,7 @@ OPS_PER_CELL = {"wta": 28, "msum": 4, "sgm_rows": 16, "sgm_cols": 16, "sgm_diag"
 # separate families).  The cost sum over 16x16 tiles writes M once (1); its
 # two-launch form (frames 0/180, then 90/270 read-modify-write) moves 3.
 BYTES_PER_CELL = {"sgm_rows": 4, "sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "sgm_vsweep": 8, "msum": 1}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch16_rig1080_d192.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch32_rig1080_d192.json")
 
 
 def log(*a):
@@ -359,7 +361,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--workload", default="rig1080", choices=["rig1080", "mixed32", "large", "serve"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rate", type=float, default=200.0)
