@@ -1411,7 +1411,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                 if (need_p) b.p[q] = ok ? LD16(part + off + 16 * q) : make_uint4(0, 0, 0, 0);
                 if (need_o) {
                     b.o[q] = ok ? LD16(g.part[0] + off + 16 * q) : make_uint4(0, 0, 0, 0);
-                    if (NO == 4) {
+                    if constexpr (NO == 4) {
                         b.o[CH + q] = ok ? LD16(g.part[1] + off + 16 * q) : make_uint4(0, 0, 0, 0);
                         b.o[2 * CH + q] = ok ? LD16(g.part[2] + off + 16 * q) : make_uint4(0, 0, 0, 0);
                         b.o[3 * CH + q] = ok ? LD16(g.rowsb + off + 16 * q) : make_uint4(0, 0, 0, 0);
