@@ -176,6 +176,34 @@ def test_serving_alone_equals_mix():
         ctx.close()
 
 
+def test_serving_fused_vertical_matches_oracle(oracle_mod):
+    """The serving loop's own plans on the fused vertical sweep (forced on for
+    the context): rigs of several widths and heights arriving together, each
+    result equal to the oracle."""
+    specs = [(200, 40, 32, 81), (96, 72, 32, 82), (64, 104, 32, 83), (152, 28, 32, 84), (80, 60, 24, 85)]
+    rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in specs]
+    ctx = sister_amd.Context()
+    try:
+        ctx.vsweep(1)
+        ctx.serve_start(3)
+        keep, pending = [], []
+        print("[serve] 5 arrivals on the fused vertical sweep", flush=True)
+        for r in rigs:
+            o = {"multiview": np.zeros((r["H"], r["W"]), np.uint16), "disp": np.zeros((r["H"], r["W"]), np.uint16)}
+            st = _host_rig_struct(r, o)
+            keep.append(st)
+            pending.append((ctx.submit(st, 1), o, r))
+        for t, o, r in pending:
+            ctx.wait(t)
+        for t, o, r in pending:
+            ref = oracle_mod.compute(r["views"], r["D"], modes=1, subpix=False)
+            assert np.array_equal(o["disp"], ref["disp"])
+            assert np.array_equal(o["multiview"], ref["multiview"])
+        ctx.serve_stop()
+    finally:
+        ctx.close()
+
+
 def test_serving_device_buffers_equal_run_device(gpu_ctx):
     """Device-resident submissions: computed in place, equal to the same rigs
     run directly."""
