@@ -12,9 +12,10 @@ inputs and outputs resident in HBM before the timed region:
 `value` is the batched throughput in G disparity-evals/s (sum of W*H*D over
 the rigs of a step / step time); the plain figures and the ratio sit beside
 it.  Per-kernel device times come from HIP events on the library stream
-(instrumented eager steps of the same batch before the graph is captured), and the
-roofline entry prices the dominant kernel's algorithmic integer work against
-the MI355X VALU peak.  The CPU baseline times the reference's own kernels
+(instrumented eager steps of the same batch before the graph is captured).
+The roofline entry prices the dominant kernel's algorithmic bytes against the
+MI355X HBM peak (and, in roofline.valu, its integer work against the VALU
+peak).  The CPU baseline times the reference's own kernels
 (oracle/_ref, census.cpp / sgm.cpp / postprocess.cpp) on one rig.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--workload rig1080|mixed32|large|serve]
