@@ -136,6 +136,91 @@ def sgm(msum):
     return out
 
 
+def lrcheck(L, R, thr=5):
+    """doLRCheck (postprocess.cpp:318-341) on copies; returns the checked L."""
+    L = np.array(L, dtype=np.float32, copy=True)
+    R = np.array(R, dtype=np.float32, copy=True)
+    lib().oracle_lrcheck(_ptr(L, _f32p), _ptr(R, _f32p), L.shape[0], L.shape[1], thr)
+    return L
+
+
+class _Kernels:
+    """The stage kernels of doMultiStereo one by one, on numpy arrays, from
+    either the oracle's restatement or the reference's compiled kernels
+    (oracle/_ref).  Glue written outside orchestrate.c drives these."""
+
+    def __init__(self, use_ref):
+        self.use_ref = use_ref
+        if use_ref:
+            l = ref_lib()
+            l.ref_cost.argtypes = [_u8p, _u8p, C.c_int, C.c_int, C.c_int, _u16p]
+            l.ref_wta_left.argtypes = [_u16p, C.c_int, C.c_int, C.c_int, _f32p]
+            l.ref_wta_right.argtypes = [_u16p, C.c_int, C.c_int, C.c_int, _f32p]
+            l.ref_median.argtypes = [_f32p, C.c_int, C.c_int]
+            l.ref_lrcheck.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int]
+            l.ref_sgm.argtypes = [_u16p, _u8p, C.c_int, C.c_int, C.c_int, _u16p]
+            self.l = l
+
+    def census(self, img):
+        return census(img, use_ref=self.use_ref)
+
+    def cost(self, imc, imn, D):
+        """ad_census (census.cpp:149-158) of two frame images; rows R-2, R-1
+        of the volume are never written and stay 0 (zeroed allocation)."""
+        imc = np.ascontiguousarray(imc, dtype=np.uint8)
+        imn = np.ascontiguousarray(imn, dtype=np.uint8)
+        if not self.use_ref:
+            return cost(census(imc), census(imn), D)
+        R, Cc = imc.shape
+        out = np.zeros((R, Cc, D), np.uint16)
+        self.l.ref_cost(_ptr(imc, _u8p), _ptr(imn, _u8p), R, Cc, D, _ptr(out, _u16p))
+        return out
+
+    def _wta(self, dsi, right):
+        if not self.use_ref:
+            return wta_right(dsi) if right else wta_left(dsi)
+        dsi = np.array(dsi, dtype=np.uint16, copy=True)
+        R, Cc, D = dsi.shape
+        out = np.empty((R, Cc), np.float32)
+        (self.l.ref_wta_right if right else self.l.ref_wta_left)(_ptr(dsi, _u16p), R, Cc, D, _ptr(out, _f32p))
+        return out
+
+    def wta_left(self, dsi):
+        return self._wta(dsi, False)
+
+    def wta_right(self, dsi):
+        return self._wta(dsi, True)
+
+    def median_inplace(self, a):
+        if not self.use_ref:
+            return median_inplace(a)
+        a = np.array(a, dtype=np.float32, copy=True)
+        self.l.ref_median(_ptr(a, _f32p), a.shape[0], a.shape[1])
+        return a
+
+    def lrcheck(self, L, R, thr=5):
+        if not self.use_ref:
+            return lrcheck(L, R, thr)
+        L = np.array(L, dtype=np.float32, copy=True)
+        R = np.array(R, dtype=np.float32, copy=True)
+        self.l.ref_lrcheck(_ptr(L, _f32p), _ptr(R, _f32p), L.shape[0], L.shape[1], thr)
+        return L
+
+    def sgm(self, msum, img):
+        if not self.use_ref:
+            return sgm(msum)
+        msum = np.array(msum, dtype=np.uint16, copy=True)
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        H, W, D = msum.shape
+        out = np.zeros((H, W, D), np.uint16)
+        self.l.ref_sgm(_ptr(msum, _u16p), _ptr(img, _u8p), H, W, D, _ptr(out, _u16p))
+        return out
+
+
+def kernels(use_ref=False):
+    return _Kernels(use_ref)
+
+
 def compute(views, D, modes=7, debug=False, use_ref=False, subpix=True):
     """Run compute_disparities on the CPU.
 

@@ -61,3 +61,21 @@ int ref_compute(const uint8_t* const views[5], int W, int H, int stride, int D, 
 
 extern "C" __attribute__((visibility("default")))
 void ref_census(const uint8_t* img, int rows, int cols, uint64_t* code) { r_census(img, rows, cols, code); }
+
+/* The stage kernels one by one, for glue written elsewhere (the numpy
+   restatement of the helper class in tests/glue_np.py drives these). */
+extern "C" __attribute__((visibility("default")))
+void ref_cost(const uint8_t* imc, const uint8_t* imn, int rows, int cols, int D, uint16_t* dsi)
+{
+    r_cost(imc, imn, rows, cols, D, dsi);
+}
+extern "C" __attribute__((visibility("default")))
+void ref_wta_left(uint16_t* dsi, int rows, int cols, int D, float* out) { r_wta_left(dsi, rows, cols, D, out); }
+extern "C" __attribute__((visibility("default")))
+void ref_wta_right(uint16_t* dsi, int rows, int cols, int D, float* out) { r_wta_right(dsi, rows, cols, D, out); }
+extern "C" __attribute__((visibility("default")))
+void ref_median(float* buf, int rows, int cols) { r_median(buf, rows, cols); }
+extern "C" __attribute__((visibility("default")))
+void ref_lrcheck(float* L, float* R, int rows, int cols, int thr) { r_lrcheck(L, R, rows, cols, thr); }
+extern "C" __attribute__((visibility("default")))
+void ref_sgm(uint16_t* dsi, const uint8_t* img, int H, int W, int D, uint16_t* S) { r_sgm(dsi, img, H, W, D, S); }

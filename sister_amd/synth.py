@@ -120,3 +120,20 @@ def rig_small(W, H, D, seed, n_boxes=2, blur_sigma=1.0):
     """Small rig for parity tests."""
     return rig_scene(W, H, D, seed, n_boxes=n_boxes, blur_sigma=blur_sigma,
                      box_side=(max(4, min(W, H) // 8), max(8, min(W, H) // 3)))
+
+
+def rig_shift(W, H, D, d, seed, blur_sigma=1.0):
+    """Rig of one constant integer disparity d with no noise: the five views
+    are crops of one textured image shifted by exactly d pixels (right and
+    left along x, top and bottom along y, in the convention above), so every
+    pixel away from the borders has its true match at d in all four views."""
+    if not 0 <= d < D:
+        raise ValueError("d must lie in [0, D)")
+    rng = np.random.default_rng(seed)
+    m = d + 1
+    T = _texture(rng, W + 2 * m, H + 2 * m, blur_sigma).astype(np.uint8)
+    crop = lambda dy, dx: np.ascontiguousarray(T[m + dy:m + dy + H, m + dx:m + dx + W])
+    # right(x, y) = C(x + d, y), top(x, y) = C(x, y - d), left(x, y) = C(x - d, y), bottom(x, y) = C(x, y + d)
+    views = [crop(0, 0), crop(0, d), crop(-d, 0), crop(0, -d), crop(d, 0)]
+    gt = np.full((H, W), float(d), np.float32)
+    return dict(views=views, gt=gt, D=D, W=W, H=H, seed=seed)
