@@ -73,6 +73,11 @@ typedef struct {
 /* hpp:22 + allocation of the device state */
 int sister_create(const sister_config* cfg, sister_ctx** out);
 void sister_destroy(sister_ctx* ctx);
+/* Text of the calling thread's last failed call on this context (like errno:
+   callers on several threads each read their own); a thread with no failure
+   of its own reads the context's last one.  The serving thread's failures are
+   reported through the tickets they fail.  The pointer stays valid until the
+   calling thread calls sister_last_error again. */
 const char* sister_last_error(const sister_ctx* ctx);
 
 /* compute_disparities (hpp:26-119) for one rig with HOST buffers: uploads,
@@ -101,7 +106,7 @@ int sister_graph_launch(sister_ctx* ctx);
 /* ---- serving: continuous batching by HBM footprint ---------------------- */
 typedef struct {
     uint64_t submitted, completed, rejected;
-    double p50_ms, p99_ms, mean_ms;   /* submit -> result ready */
+    double p50_ms, p99_ms, mean_ms;   /* submit -> result ready, over the last 65536 completions */
     double rigs_per_s;                /* completed / (last completion - first submit) */
     size_t hbm_in_use, hbm_peak;
 } sister_stats;
@@ -118,14 +123,18 @@ int sister_submit(sister_ctx* ctx, const sister_rig* rig, int modes, uint64_t* t
    the rig's device buffers, which must stay valid until completion.  The
    footprint charged against the budget is the workspace alone. */
 int sister_submit_device(sister_ctx* ctx, const sister_rig* rig, int modes, uint64_t* ticket);
+/* A ticket stays known until 65536 later requests have finished; after that
+   poll and wait report SISTER_ENOTFOUND for it. */
 int sister_poll(sister_ctx* ctx, uint64_t ticket, int* done);
 int sister_wait(sister_ctx* ctx, uint64_t ticket);
 int sister_serve_stop(sister_ctx* ctx);
 int sister_stats_get(sister_ctx* ctx, sister_stats* out);
 
 /* ---- instrumentation and test access ----------------------------------- */
-/* Per-stage device time (HIP events on the context stream) accumulated while
-   enabled.  names[i] / ms[i] / launches[i] for i < returned count. */
+/* Per-kernel device time (HIP events around each kernel's launches on the
+   stream they run on) accumulated while enabled: names[i] / ms[i] /
+   launches[i] (kernel dispatches issued) for i < returned count.  Profiled
+   runs issue the SGM families one after another. */
 int sister_profile_enable(sister_ctx* ctx, int on);
 int sister_profile_read(sister_ctx* ctx, const char** names, double* ms, int* launches, int max);
 /* Copy an intermediate of rig `index` of the last run to host memory:

@@ -62,6 +62,7 @@ struct KernelTimer {
     virtual void begin(int kernel, hipStream_t st) = 0;
     virtual void end(int kernel, hipStream_t st) = 0;
     virtual ~KernelTimer() {}
+    long dispatched[kNumKernels] = {};   // kernel dispatches issued, per kernel
 };
 
 bool wta_rows_ok(int maxCols, int maxD);

@@ -2178,11 +2178,13 @@ const char* const kKernelNames[kNumKernels] = {
 #define TE(id) do { if (t) t->end(id, st); } while (0)
 #define TB_S(id, s) do { if (t) t->begin(id, s); } while (0)
 #define TE_S(id, s) do { if (t) t->end(id, s); } while (0)
+#define TD(id, n) do { if (t) t->dispatched[id] += (n); } while (0)
 
 void launch_census(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
 {
     TB(kKCensus);
     hipLaunchKernelGGL(k_census, dim3(ls.blocks[kStageCensus]), dim3(256), 0, st, ls.rigs, ls.nrig, ls.bs[kStageCensus]);
+    TD(kKCensus, 1);
     TE(kKCensus);
 }
 
@@ -2197,9 +2199,11 @@ void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
                            wta_row_lds(ls.maxWp, ls.maxD), st, ls.rigs, ls.nrig, ls.bs[kStageWta]);
         hipLaunchKernelGGL(k_wta_row<true>, dim3(ls.blocks[kStageWtaV]), dim3(64 * waves13),
                            wta_row_lds(ls.maxHp, ls.maxD), st, ls.rigs, ls.nrig, ls.bs[kStageWtaV]);
+        TD(kKWta, 2);
     } else {
         size_t lds = (size_t)2 * (256 + ls.maxD - 1) * sizeof(uint64_t);
         hipLaunchKernelGGL(k_wta, dim3(ls.blocks[kStageWta]), dim3(256), lds, st, ls.rigs, ls.nrig, ls.bs[kStageWta]);
+        TD(kKWta, 1);
     }
     TE(kKWta);
 }
@@ -2221,12 +2225,14 @@ void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
         TB(kKMedianPrep);
         hipLaunchKernelGGL(k_median_prep, dim3(ls.blocks[kStageMedianPrep]), dim3(256), 0, st, ls.rigs, ls.nrig,
                            ls.bs[kStageMedianPrep], ls.scratch);
+        TD(kKMedianPrep, 1);
         TE(kKMedianPrep);
     }
     TB(kKMedianSerial);
     if (raw) hipLaunchKernelGGL(k_median_serial<0>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
     else if (wide) hipLaunchKernelGGL(k_median_serial<2>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
     else hipLaunchKernelGGL(k_median_serial<1>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
+    TD(kKMedianSerial, 1);
     TE(kKMedianSerial);
 }
 
@@ -2234,6 +2240,7 @@ void launch_mask(const LaunchSet& ls, int fmask, hipStream_t st, KernelTimer* t)
 {
     TB(kKMask);
     hipLaunchKernelGGL(k_mask, dim3(ls.blocks[kStageMask]), dim3(256), 0, st, ls.rigs, ls.nrig, ls.bs[kStageMask], fmask);
+    TD(kKMask, 1);
     TE(kKMask);
 }
 
@@ -2341,10 +2348,12 @@ void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t)
         const DGroup& G = ls.groups[gi];
         if ((fuse & 5) && (fuse & 10) && msum_tile_fits(G) && !msum_split_forced()) {
             launch_msum_tile(G, fuse, st);                 // all frames, M written once
+            TD(kKMsum, 1);
             continue;
         }
         if (fuse & 5) launch_msum_side<0>(G, fuse, st);    // frames 0/180 write M (zero when alone)
         if (fuse & 10) launch_msum_side<1>(G, fuse, st);   // frames 90/270 add (or write when alone)
+        TD(kKMsum, ((fuse & 5) ? 1 : 0) + ((fuse & 10) ? 1 : 0));
     }
     TE(kKMsum);
 }
@@ -2382,7 +2391,7 @@ int vsweep_blocks(const RigDesc& g, int vcols)
     return vcols > 0 ? (g.Wp + vcols - 1) / vcols : 0;
 }
 
-static void launch_vsweep(const DGroup& G, int mode, int keep_sum, hipStream_t st)
+static int launch_vsweep(const DGroup& G, int mode, int keep_sum, hipStream_t st)
 {
     const int nt = G.vcols + 2 * kVGhost;
     const size_t lds = vs_lds(G.D, G.vcols);
@@ -2420,6 +2429,7 @@ static void launch_vsweep(const DGroup& G, int mode, int keep_sum, hipStream_t s
             }
 #undef SISTER_VS
         }
+    return 2 * nseg;
 }
 
 void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, const FamStreams* fs, KernelTimer* t)
@@ -2432,9 +2442,10 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
             const DGroup& G = ls.groups[gi];
             TB_S(kKRows, st);
             SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 0);
+            TD(kKRows, 1);
             TE_S(kKRows, st);
             TB_S(kKVSweep, st);
-            launch_vsweep(G, mode, keep_sum, st);
+            TD(kKVSweep, launch_vsweep(G, mode, keep_sum, st));
             TE_S(kKVSweep, st);
         }
         return;
@@ -2455,17 +2466,21 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         if (G.vcols > 0) continue;
         TB_S(kKRows, st);
         SISTER_FAM(G, 0, dim3(2 * G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 1);
+        TD(kKRows, 1);
         TE_S(kKRows, st);
         TB_S(kKCols, sc);
         SISTER_FAM(G, 1, dim3(G.blocks[1]), dim3(64), 0, sc, G.rigs, G.nrig, G.bs[1], mode, keep_sum, 0);
+        TD(kKCols, 1);
         TE_S(kKCols, sc);
         TB_S(kKDiag, sd);
         SISTER_FAM(G, 2, dim3(G.blocks[2]), dim3(64), 0, sd, G.rigs, G.nrig, G.bs[2], mode, keep_sum, 0);
+        TD(kKDiag, 1);
         TE_S(kKDiag, sd);
         // the anti-diagonal family's forward sweep needs M alone: it follows
         // the diagonals on their stream, beside the rows
         TB_S(kKAnti, sd);
         SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), 0, sd, G.rigs, G.nrig, G.bs[3], mode, keep_sum, 1);
+        TD(kKAnti, 1);
         TE_S(kKAnti, sd);
     }
     if (fs) {
@@ -2479,15 +2494,17 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
         if (G.vcols > 0) {
             TB_S(kKRows, st);
             SISTER_FAM(G, 0, dim3(G.blocks[0]), dim3(64), 0, st, G.rigs, G.nrig, G.bs[0], mode, keep_sum, 0);
+            TD(kKRows, 1);
             TE_S(kKRows, st);
             TB_S(kKVSweep, st);
-            launch_vsweep(G, mode, keep_sum, st);
+            TD(kKVSweep, launch_vsweep(G, mode, keep_sum, st));
             TE_S(kKVSweep, st);
             continue;
         }
         const size_t lds = (size_t)64 * G.CH[3] * 32;
         TB_S(kKAnti, st);
         SISTER_FAM(G, 3, dim3(G.blocks[3]), dim3(64), lds, st, G.rigs, G.nrig, G.bs[3], mode, keep_sum, 2);
+        TD(kKAnti, 1);
         TE_S(kKAnti, st);
     }
 }

@@ -175,7 +175,8 @@ struct Plan {
 struct sister_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    std::string err;
+    std::string err;           // last error of a caller thread (guarded by emu)
+    mutable std::mutex emu;
     Pool pool;
     EventTimer timer;
     bool profiling = false;
@@ -217,20 +218,63 @@ struct sister_ctx {
     std::condition_variable scv;
     std::deque<Request> queue;
     std::unordered_map<uint64_t, int> status;   // 0 queued, 1 running, 2 done, 3 failed
+    std::deque<uint64_t> finished;               // finished tickets, oldest first
     std::string serve_error;                     // reason of the last failed request
     bool serving = false, stop = false;
     int max_inflight = 4;
     uint64_t next_ticket = 1;
+    // latencies of the last kLatWindow completions (a ring)
+    static constexpr size_t kLatWindow = 1 << 16;
+    static constexpr size_t kKeepFinished = 1 << 16;   // finished tickets remembered
     std::vector<double> lat_ms;
+    size_t lat_next = 0;
     uint64_t n_sub = 0, n_done = 0, n_rej = 0;
     std::chrono::steady_clock::time_point t_first{}, t_last{};
     bool have_first = false;
 };
 
+// The serving thread points this at its own string, so that a plan it fails
+// to build never writes the context's error (which caller threads own).
+static thread_local std::string* tl_err_sink = nullptr;
+// The calling thread's last failure and the context it came from: what
+// sister_last_error returns to that thread (like errno), so that callers on
+// several threads each read their own failure.
+static thread_local const sister_ctx* tl_err_ctx = nullptr;
+static thread_local std::string tl_err;
+
 static int fail(sister_ctx* c, int code, const std::string& msg)
 {
-    if (c) c->err = msg;
+    if (tl_err_sink) {
+        *tl_err_sink = msg;
+    } else if (c) {
+        tl_err_ctx = c;
+        tl_err = msg;
+        std::lock_guard<std::mutex> lk(c->emu);
+        c->err = msg;
+    }
     return code;
+}
+
+// Under smu: a request's final state (2 done, 3 failed).  The table keeps the
+// last kKeepFinished finished tickets; older ones are forgotten.
+static void finish_ticket(sister_ctx* c, uint64_t t, int st)
+{
+    c->status[t] = st;
+    c->finished.push_back(t);
+    while (c->finished.size() > sister_ctx::kKeepFinished) {
+        c->status.erase(c->finished.front());
+        c->finished.pop_front();
+    }
+}
+
+static void record_latency(sister_ctx* c, double ms)
+{
+    if (c->lat_ms.size() < sister_ctx::kLatWindow) {
+        c->lat_ms.push_back(ms);
+    } else {
+        c->lat_ms[c->lat_next] = ms;
+        c->lat_next = (c->lat_next + 1) % sister_ctx::kLatWindow;
+    }
 }
 
 #define HIPCHK(ctx, expr)                                                                        \
@@ -509,6 +553,7 @@ int sister_create(const sister_config* cfg, sister_ctx** out)
 void sister_destroy(sister_ctx* c)
 {
     if (!c) return;
+    if (tl_err_ctx == c) tl_err_ctx = nullptr;
     sister_serve_stop(c);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
@@ -535,7 +580,18 @@ void sister_destroy(sister_ctx* c)
     delete c;
 }
 
-const char* sister_last_error(const sister_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* sister_last_error(const sister_ctx* c)
+{
+    // the calling thread's own last failure on this context; otherwise a
+    // copy of the context's last failure (from any thread).  Either pointer
+    // stays valid until this thread's next call.
+    static thread_local std::string copy;
+    if (!c) return "null context";
+    if (tl_err_ctx == c) return tl_err.c_str();
+    std::lock_guard<std::mutex> lk(c->emu);
+    copy = c->err;
+    return copy.c_str();
+}
 
 void* sister_stream(sister_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
@@ -698,7 +754,7 @@ int sister_profile_enable(sister_ctx* c, int on)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->timer.collect();
     if (on) {
-        for (int i = 0; i < kNumKernels; i++) { c->timer.ms[i] = 0; c->timer.count[i] = 0; }
+        for (int i = 0; i < kNumKernels; i++) { c->timer.ms[i] = 0; c->timer.count[i] = 0; c->timer.dispatched[i] = 0; }
     }
     c->profiling = on != 0;
     return SISTER_OK;
@@ -713,7 +769,7 @@ int sister_profile_read(sister_ctx* c, const char** names, double* ms, int* laun
     for (int i = 0; i < n; i++) {
         if (names) names[i] = kKernelNames[i];
         if (ms) ms[i] = c->timer.ms[i];
-        if (launches) launches[i] = c->timer.count[i];
+        if (launches) launches[i] = (int)c->timer.dispatched[i];
     }
     return n;
 }
@@ -772,6 +828,8 @@ static size_t rig_footprint(const sister_rig& r, bool device)
 
 static void serve_loop(sister_ctx* c)
 {
+    std::string plan_error;   // this thread's failures (see fail())
+    tl_err_sink = &plan_error;
     (void)hipSetDevice(c->device);
     std::vector<sister_ctx::Inflight*> inflight;
     std::vector<hipStream_t> streams;
@@ -816,9 +874,9 @@ static void serve_loop(sister_ctx* c)
             {
                 std::lock_guard<std::mutex> lk(c->smu);
                 for (auto& r : f->reqs) {
-                    c->status[r.ticket] = (q == hipSuccess) ? 2 : 3;
+                    finish_ticket(c, r.ticket, (q == hipSuccess) ? 2 : 3);
                     if (q != hipSuccess) c->serve_error = hipGetErrorString(q);
-                    c->lat_ms.push_back(std::chrono::duration<double, std::milli>(now - r.t_submit).count());
+                    record_latency(c, std::chrono::duration<double, std::milli>(now - r.t_submit).count());
                     reserved -= r.footprint;
                     c->n_done++;
                 }
@@ -841,7 +899,7 @@ static void serve_loop(sister_ctx* c)
                     if (reserved + r.footprint > c->pool.budget) {
                         if (inflight.empty() && batch.empty()) {   // can never fit
                             c->serve_error = "footprint exceeds the HBM budget";
-                            c->status[r.ticket] = 3;
+                            finish_ticket(c, r.ticket, 3);
                             c->n_rej++;
                             c->queue.pop_front();
                             c->scv.notify_all();
@@ -922,7 +980,7 @@ static void serve_loop(sister_ctx* c)
                 const int rc = build_plan(c, dev.data(), (int)dev.size(), f->plan);
                 ok = rc == SISTER_OK;
                 no_memory = rc == SISTER_ENOMEM;
-                if (!ok) fail_reason = c->err;
+                if (!ok) fail_reason = plan_error;
             } else {
                 fail_reason = "staging allocation failed";
             }
@@ -963,7 +1021,7 @@ static void serve_loop(sister_ctx* c)
                 }
             } else {
                 c->serve_error = fail_reason;
-                for (auto& r : f->reqs) { c->status[r.ticket] = 3; reserved -= r.footprint; c->n_rej++; }
+                for (auto& r : f->reqs) { finish_ticket(c, r.ticket, 3); reserved -= r.footprint; c->n_rej++; }
             }
             c->scv.notify_all();
             (void)hipEventDestroy(f->done);
@@ -988,6 +1046,7 @@ int sister_serve_start(sister_ctx* c, int max_inflight)
     c->stop = false;
     c->serving = true;
     c->lat_ms.clear();
+    c->lat_next = 0;
     c->n_sub = c->n_done = c->n_rej = 0;
     c->have_first = false;
     c->server = std::thread(serve_loop, c);
@@ -1043,7 +1102,7 @@ int sister_poll(sister_ctx* c, uint64_t t, int* done)
     if (!c || !done) return SISTER_EINVAL;
     std::lock_guard<std::mutex> lk(c->smu);
     auto it = c->status.find(t);
-    if (it == c->status.end()) return fail(c, SISTER_ENOTFOUND, "unknown ticket");
+    if (it == c->status.end()) return fail(c, SISTER_ENOTFOUND, "unknown ticket (never issued, or forgotten)");
     *done = it->second >= 2;
     return it->second == 3 ? fail(c, SISTER_EDEVICE, "request failed: " + c->serve_error) : SISTER_OK;
 }
@@ -1076,12 +1135,20 @@ int sister_serve_stop(sister_ctx* c)
 int sister_stats_get(sister_ctx* c, sister_stats* s)
 {
     if (!c || !s) return SISTER_EINVAL;
-    std::lock_guard<std::mutex> lk(c->smu);
     std::memset(s, 0, sizeof(*s));
-    s->submitted = c->n_sub;
-    s->completed = c->n_done;
-    s->rejected = c->n_rej;
-    std::vector<double> v = c->lat_ms;
+    std::vector<double> v;
+    {
+        std::lock_guard<std::mutex> lk(c->smu);
+        s->submitted = c->n_sub;
+        s->completed = c->n_done;
+        s->rejected = c->n_rej;
+        v = c->lat_ms;
+        if (c->have_first && c->n_done > 0) {
+            double span = std::chrono::duration<double>(c->t_last - c->t_first).count();
+            if (span > 0) s->rigs_per_s = c->n_done / span;
+        }
+    }
+    // percentiles outside the lock the serving thread needs
     if (!v.empty()) {
         std::sort(v.begin(), v.end());
         auto pct = [&](double p) { size_t i = (size_t)std::min<double>(v.size() - 1, std::floor(p * (v.size() - 1) + 0.5)); return v[i]; };
@@ -1091,10 +1158,7 @@ int sister_stats_get(sister_ctx* c, sister_stats* s)
         for (double x : v) sum += x;
         s->mean_ms = sum / v.size();
     }
-    if (c->have_first && c->n_done > 0) {
-        double span = std::chrono::duration<double>(c->t_last - c->t_first).count();
-        if (span > 0) s->rigs_per_s = c->n_done / span;
-    }
+    std::lock_guard<std::mutex> lk(c->mu);   // the arena is mutated under mu
     s->hbm_in_use = c->pool.in_use;
     s->hbm_peak = c->pool.peak;
     return SISTER_OK;

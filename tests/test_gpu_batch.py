@@ -330,3 +330,59 @@ def test_full_1080_d192_fused_vertical_equals_separate(gpu_ctx):
         gpu_ctx.vsweep(-1)
     for k in ("multiview", "horizontal", "vertical", "disp", "subpix"):
         assert np.array_equal(fused[k], sep[k]), (k, int((fused[k] != sep[k]).sum()))
+
+
+def test_serving_caller_errors_while_serving():
+    """Caller threads fail calls (invalid submissions, unknown tickets) and
+    read the error text while the serving thread builds and retires plans:
+    each caller reads its own failure's text, every request completes, and
+    results equal the same rigs alone."""
+    import threading
+    import ctypes as C
+    rigs = [synth.rig_small(W, H, D, s) for W, H, D, s in MIX]
+    ctx = sister_amd.Context()
+    try:
+        singles = [ctx.compute_rig(r["views"], r["D"], modes=1, subpix=False) for r in rigs]
+        ctx.serve_start(3)
+        L = sister_amd.lib()
+        errors = []
+
+        def hammer(seed):
+            bad = sister_amd.RigStruct()
+            bad.width, bad.height, bad.stride, bad.disp_count = 66, 48, 66, 16   # W not a multiple of 4
+            t = C.c_uint64()
+            for i in range(3000):
+                if i % 2:
+                    rc = L.sister_submit(ctx._h, C.byref(bad), 1, C.byref(t))
+                    want, text = sister_amd.SISTER_EINVAL, b"invalid rig shape"
+                else:
+                    d = C.c_int()
+                    rc = L.sister_poll(ctx._h, 10**12 + seed * 10**6 + i, C.byref(d))
+                    want, text = sister_amd.SISTER_ENOTFOUND, b"unknown ticket"
+                msg = L.sister_last_error(ctx._h)
+                if rc != want or not msg.startswith(text):
+                    errors.append((seed, i, rc, msg))
+                    return
+
+        threads = [threading.Thread(target=hammer, args=(k,)) for k in range(2)]
+        for th in threads:
+            th.start()
+        print("[serve] 24 requests while two threads fail calls", flush=True)
+        pending = []
+        for i in range(24):
+            r = rigs[i % len(rigs)]
+            o = {"multiview": np.zeros((r["H"], r["W"]), np.uint16), "disp": np.zeros((r["H"], r["W"]), np.uint16)}
+            s = _host_rig_struct(r, o)
+            pending.append((ctx.submit(s, 1), o, s, i % len(rigs)))
+        for t, o, s, k in pending:
+            ctx.wait(t)
+        for th in threads:
+            th.join()
+        assert not errors, errors[:3]
+        for t, o, s, k in pending:
+            assert np.array_equal(o["disp"], singles[k]["disp"]), k
+        st = ctx.stats()
+        assert st["completed"] == 24
+        ctx.serve_stop()
+    finally:
+        ctx.close()
