@@ -11,21 +11,40 @@ inputs and outputs resident in HBM before the timed region:
 
 `value` is the batched throughput in G disparity-evals/s (sum of W*H*D over
 the rigs of a step / step time); the plain figures and the ratio sit beside
-it.  Per-kernel device times come from HIP events on the library stream
-(instrumented eager steps of the same batch before the graph is captured).
-The roofline entry prices the dominant kernel's algorithmic bytes against the
-MI355X HBM peak (and, in roofline.valu, its integer work against the VALU
-peak).  The CPU baseline times the reference's own kernels
-(oracle/_ref, census.cpp / sgm.cpp / postprocess.cpp) on one rig.
+it, and every rig of the batch is compared with its plain result.
 
-    python bench.py [--steps K] [--warmup W] [--batch B] [--workload rig1080|mixed32|large|serve]
+`roofline` prices the whole step by its algorithmic integer work: 97 int
+lane-ops per padded cell (W'*H'*D) against the MI355X VALU peak, with the HBM
+bytes the counters measured per step beside it.  Its `dominant_kernel` entry
+does the same for the kernel with the most device time: its own
+ops per cell, its average launch duration (HIP events around its launches on
+the library stream, in instrumented eager steps of the same batch taken
+before the graph is captured; real dispatch counts), and its counter traffic
+against its algorithmic bytes.
+
+Two more blocks run after the headline, each in a fresh context:
+  mixed32  32 rigs of mixed sizes (640x480 .. 2048x1536) and D (64 .. 256)
+           captured as one graph;
+  serve    Poisson arrivals at 200 rigs/s from the same mix plus 5%
+           4096x3000 rigs, HBM budget 64 GiB: rigs/s, p50/p99 latency.
+
+The CPU baseline times the reference's own kernels (oracle/_ref:
+census.cpp / sgm.cpp / postprocess.cpp behind the restated glue) on the
+host: with the reference's parallelism (OpenMP 4 threads in the cost stage),
+one warm-up and the median of three runs of one 1920x1080 D=192 rig; and on
+every core, one rig-sized band per core computed concurrently.
+
+    python bench.py [--steps K] [--warmup W] [--batch B]
+                    [--workload rig1080|mixed32|large|serve] [--no-extra]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import statistics
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -35,6 +54,7 @@ METRIC = ("G disparity-evals/s and rigs/s (p50/p99 latency), batched vs plain, "
           "1 MI355X, bit-exact")
 VALU_PEAK = 256 * 64 * 2.4e9      # int32 lane-ops/s: 256 CUs x 64 lanes/clk x 2.4 GHz
 HBM_PEAK = 8.0e12
+OPS_PER_CELL_PIPELINE = 97        # mode 0: costs 20, LR-check argmins 8, weighted sum 4, SGM 64, WTA 1
 
 # Algorithmic integer lane-ops per padded cell (W'*H'*D) of mode 0, split per
 # kernel: 4 directional costs (xor 2 + popcount 2 + invalid select 1 each = 20)
@@ -59,6 +79,7 @@ OPS_PER_CELL = {"wta": 28, "msum": 4, "sgm_rows": 16, "sgm_cols": 16, "sgm_diag"
 # two-launch form (frames 0/180, then 90/270 read-modify-write) moves 3.
 BYTES_PER_CELL = {"sgm_rows": 4, "sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "sgm_vsweep": 8, "msum": 1}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch48_rig1080_d192.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_batch48_rig1080_d192.json")
 
 
 def log(*a):
@@ -94,31 +115,114 @@ def to_device(ctx, rigs, torch):
     return dev
 
 
+# ---------------------------------------------------------------------------
+# CPU baseline
+# ---------------------------------------------------------------------------
+def host_description():
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemAvailable"):
+                    avail = int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    return {"cpus_usable": cores, "cpus_online": os.cpu_count(), "model": model, "mem_available": avail}
+
+
 def cpu_baseline(rig):
-    """Reference kernels (oracle/_ref) on one rig, mode 0; the scalar port
-    when the reference build is absent."""
+    """The reference's kernels (oracle/_ref) behind the restated glue, mode 0,
+    timed on the host.  Reference parallelism: OpenMP 4 threads in the cost
+    stage only (census.cpp:116-139, numThreads=4 at hpp:77), every other
+    stage single-threaded; one warm-up on a 640x480 D=64 rig, then the median
+    of three runs of the 1920x1080 D=192 rig.  All cores: one band of the
+    same rig (all 1920 columns, 128 rows, D=192) per usable core, computed
+    concurrently (a rig per core, the batch's own parallelism), scaled to
+    the full rig's evals per padded cell.  The scalar port stands in when the
+    reference build is absent."""
     from oracle import oracle as O
+    from sister_amd import synth
     kind = "reference" if O.ref_available() else "port"
+    use_ref = kind == "reference"
+    host = host_description()
+    H, W, D = rig["H"], rig["W"], rig["D"]
+    warm = synth.rig_small(640, 480, 64, 1)
     t = time.perf_counter()
-    O.compute(rig["views"], rig["D"], modes=1, use_ref=(kind == "reference"), subpix=False)
-    dt = time.perf_counter() - t
-    H, W = rig["views"][0].shape
-    return {
-        "value": W * H * rig["D"] / dt / 1e9,
+    O.compute(warm["views"], 64, modes=1, use_ref=use_ref, subpix=False)
+    warm_s = time.perf_counter() - t
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        O.compute(rig["views"], D, modes=1, use_ref=use_ref, subpix=False)
+        times.append(time.perf_counter() - t)
+    med = statistics.median(times)
+    evals = W * H * D
+    evals_per_cell = evals / padded_cells(W, H, D)
+    out = {
+        "value": evals / med / 1e9,
         "unit": "G disparity-evals/s",
-        "cores": 4 if kind == "reference" else 1,
+        "cores": 4 if use_ref else 1,
         "kind": kind,
-        "sample": (f"one {W}x{H} D={rig['D']} rig, doMultiStereo mode 0 (4 cost volumes, masks, "
-                   f"weighted sum, SGM, WTA) in {dt:.2f} s wall; the reference's OpenMP uses 4 threads "
-                   f"in the cost stage only"),
-        "seconds": dt,
+        "sample": (f"one {W}x{H} D={D} rig, doMultiStereo mode 0 (4 cost volumes, masks, weighted sum, SGM, "
+                   f"WTA), median of 3 runs after a 640x480 D=64 warm-up; the reference's OpenMP uses 4 threads "
+                   f"in the cost stage only, the other stages one"),
+        "seconds": times,
+        "warmup_seconds": warm_s,
+        "host": host,
     }
+    # all cores: bands of the same scene, one per usable core, bounded by memory
+    bh = 128
+    per_copy = padded_cells(W, bh, D) * 2 * 9 + 16 * W * bh   # nine u16 volumes plus images
+    ncores = host["cpus_usable"] or 1
+    ncopies = min(ncores, 16)   # at most the 16 cores a one-GPU share of the host holds
+    if host["mem_available"]:
+        ncopies = max(1, min(ncores, int(host["mem_available"] * 0.4 // per_copy)))
+    band = [v[:bh] for v in rig["views"]]
+    t_each = [0.0] * ncopies
+
+    def work(k):
+        t0 = time.perf_counter()
+        O.compute(band, D, modes=1, use_ref=use_ref, subpix=False)
+        t_each[k] = time.perf_counter() - t0
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(ncopies)]
+    t0 = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    wall = time.perf_counter() - t0
+    cells_rate = ncopies * padded_cells(W, bh, D) / wall
+    out["all_cores"] = {
+        "value": cells_rate * evals_per_cell / 1e9,
+        "unit": "G disparity-evals/s (at the 1920x1080 D=192 rig's evals per padded cell)",
+        "padded_cells_per_s": cells_rate,
+        "threads": ncopies,
+        "sample": (f"{ncopies} concurrent {W}x{bh} D={D} bands of the same scene, each with the reference's "
+                   f"parallelism ({ncores} usable cores; copies limited to 16, the one-GPU share of a host, and "
+                   f"to 40% of the available memory), one round after the runs above, {wall:.2f} s wall"),
+        "seconds_per_copy": [round(x, 3) for x in t_each],
+    }
+    return out
 
 
+# ---------------------------------------------------------------------------
+# profiling run (the command rocprofv3 wraps)
+# ---------------------------------------------------------------------------
 def run_single(args):
     """One instrumented eager run of the batch, the same launches whose event
-    times give kernel_ms (one dispatch per kernel and D group, families in
-    sequence): the command profiled by rocprofv3 (--kernel-trace, --pmc)."""
+    times give kernel_ms (families in sequence): the command profiled by
+    rocprofv3 (--kernel-trace, --pmc)."""
     import torch
     import sister_amd
 
@@ -133,8 +237,55 @@ def run_single(args):
     ctx.profile(False)
     cells = sum(padded_cells(r["W"], r["H"], r["D"]) for r in rigs)
     print(json.dumps({"single_run_rigs": len(rigs), "padded_cells": cells,
-                      "kernel_ms": {k: ms for k, (ms, n) in prof.items() if n}}))
+                      "kernel_ms": {k: ms for k, (ms, n) in prof.items() if n},
+                      "kernel_dispatches": {k: n for k, (ms, n) in prof.items() if n}}))
     ctx.close()
+
+
+# ---------------------------------------------------------------------------
+# headline: plain vs batched at 1920x1080 D=192
+# ---------------------------------------------------------------------------
+def roofline_entry(kern, launches, cells, step_ms, B, workload):
+    traffic = None
+    per_kernel_traffic = {}
+    valu = {}
+    if os.path.exists(TRAFFIC_FILE):
+        t = json.load(open(TRAFFIC_FILE))
+        if workload == "rig1080" and B == t.get("rigs"):
+            per_kernel_traffic = {k: v["fetch_bytes"] + v["write_bytes"] for k, v in t["kernels"].items()}
+            traffic = sum(per_kernel_traffic.values())
+    if os.path.exists(VALU_FILE):
+        v = json.load(open(VALU_FILE))
+        if workload == "rig1080" and B == v.get("rigs"):
+            valu = v.get("kernels", {})
+    step_s = step_ms * 1e-3
+    achieved = OPS_PER_CELL_PIPELINE * cells / step_s
+    roof = {"bound": "valu", "scope": "whole step (every kernel of the batch)",
+            "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12, "unit": "T int-lane-ops/s",
+            "frac": achieved / VALU_PEAK, "ops_per_cell": OPS_PER_CELL_PIPELINE,
+            "traffic": traffic,
+            "hbm_achieved": traffic / step_s / 1e9 if traffic else None,
+            "hbm_frac": traffic / step_s / HBM_PEAK if traffic else None,
+            "hbm_unit": "GB/s"}
+    dom = max(kern, key=lambda k: kern[k])
+    n = launches.get(dom) or 1
+    avg_ms = kern[dom] / n
+    ops = OPS_PER_CELL.get(dom, 0) * cells
+    d = {"kernel": dom, "bound": "valu", "ops_per_cell": OPS_PER_CELL.get(dom, 0),
+         "launches_per_step": n, "avg_launch_ms": avg_ms,
+         "achieved": ops / n / (avg_ms * 1e-3) / 1e12, "peak": VALU_PEAK / 1e12, "unit": "T int-lane-ops/s",
+         "frac": ops / n / (avg_ms * 1e-3) / VALU_PEAK}
+    if dom in BYTES_PER_CELL:
+        alg = BYTES_PER_CELL[dom] * cells
+        d.update({"bytes_per_cell": BYTES_PER_CELL[dom], "hbm_achieved": alg / (kern[dom] * 1e-3) / 1e9,
+                  "hbm_frac": alg / (kern[dom] * 1e-3) / HBM_PEAK, "hbm_unit": "GB/s (algorithmic bytes)"})
+        if dom in per_kernel_traffic:
+            d["traffic"] = per_kernel_traffic[dom]
+            d["traffic_over_algorithmic"] = per_kernel_traffic[dom] / alg
+    if dom in valu:
+        d["valu_counters"] = valu[dom]
+    roof["dominant_kernel"] = d
+    return roof
 
 
 def run(args):
@@ -149,6 +300,7 @@ def run(args):
     evals = sum(r["views"][0].shape[0] * r["views"][0].shape[1] * r["D"] for r in rigs)
     cells = sum(padded_cells(r["W"], r["H"], r["D"]) for r in rigs)
     torch.cuda.synchronize()
+    log(f"headline: {B} rigs, {cells / 1e9:.2f} G padded cells per step")
 
     # ---- plain: one rig per call ------------------------------------------
     for i in range(args.warmup):
@@ -163,17 +315,17 @@ def run(args):
     torch.cuda.synchronize()
     plain_ms = (time.perf_counter() - t0) * 1e3 / nplain
     plain_evals = sum(rigs[i % B]["W"] * rigs[i % B]["H"] * rigs[i % B]["D"] for i in range(nplain)) / nplain
-    ncmp = min(B, 4)
-    for i in range(nplain, ncmp):          # every compared rig computed alone once
+    # the distinct scenes (rig i is scene i % nscene), each computed alone
+    nscene = min(B, 4) if args.workload == "rig1080" else B
+    for i in range(nscene):
         ctx.run_device([dev[i]["struct"]], 1)
-    ctx.sync()
-    plain_out = [d["outs"]["disp"].cpu().numpy().copy() for d in dev[:ncmp]]
+        ctx.sync()
+    plain_out = [{k: dev[i]["outs"][k].clone() for k in ("disp", "multiview")} for i in range(nscene)]
+    log(f"plain: {plain_ms:.2f} ms per rig")
 
     # ---- per-kernel device time: instrumented eager steps of the same batch --
     # (before the capture; a one-rig call then replaces the batch-sized eager
     # plan, so that only the graph's workspaces are held while it is timed)
-    # (cost-sum and SGM launches run once per group of equal D: the figures
-    # are per-step totals over a kernel's launches)
     nrep = max(1, min(args.steps, 3))
     ctx.profile(True)
     for _ in range(nrep):
@@ -187,6 +339,7 @@ def run(args):
     # ---- batched: all B rigs in one captured plan ----------------------------
     for d in dev:
         d["outs"]["disp"].zero_()
+        d["outs"]["multiview"].zero_()
     ctx.graph_capture([d["struct"] for d in dev], 1)
     for _ in range(args.warmup):
         ctx.graph_launch()
@@ -198,34 +351,12 @@ def run(args):
     ctx.sync()
     torch.cuda.synchronize()
     step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    same = all(np.array_equal(dev[i]["outs"]["disp"].cpu().numpy(), plain_out[i]) for i in range(len(plain_out)))
+    log(f"batched: {step_ms:.2f} ms per step")
+    bad = [i for i in range(B)
+           if not all(torch.equal(dev[i]["outs"][k], plain_out[i % nscene][k]) for k in ("disp", "multiview"))]
 
     kern = {k: ms / nrep for k, (ms, n) in prof.items() if n}
     launches = {k: n / nrep for k, (ms, n) in prof.items() if n}
-    dom = max(kern, key=lambda k: kern[k])
-    if dom in BYTES_PER_CELL:
-        achieved = BYTES_PER_CELL[dom] * cells / (kern[dom] * 1e-3)
-        traffic = None
-        if os.path.exists(TRAFFIC_FILE):
-            t = json.load(open(TRAFFIC_FILE))
-            matches = (args.workload == "rig1080" and B == t.get("rigs"))
-            k = t["kernels"].get(dom)
-            if matches and k:
-                traffic = k["fetch_bytes"] + k["write_bytes"]
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                    "bytes_per_cell": BYTES_PER_CELL[dom], "launches_per_step": launches[dom]}
-        if dom in OPS_PER_CELL:
-            # the same kernel priced by its algorithmic integer work
-            v = OPS_PER_CELL[dom] * cells / (kern[dom] * 1e-3)
-            roofline["valu"] = {"achieved": v / 1e12, "peak": VALU_PEAK / 1e12, "unit": "T int-lane-ops/s",
-                                "frac": v / VALU_PEAK, "ops_per_cell": OPS_PER_CELL[dom]}
-    else:
-        achieved = OPS_PER_CELL.get(dom, 0) * cells / (kern[dom] * 1e-3)
-        roofline = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
-                    "unit": "T int-lane-ops/s", "frac": achieved / VALU_PEAK, "traffic": None,
-                    "ops_per_cell": OPS_PER_CELL.get(dom, 0)}
-
     out = {
         "metric": METRIC,
         "value": evals / (step_ms * 1e-3) / 1e9,
@@ -251,21 +382,59 @@ def run(args):
         "plain": {"ms_per_rig": plain_ms, "rigs_per_s": 1e3 / plain_ms,
                   "value": plain_evals / (plain_ms * 1e-3) / 1e9},
         "batched_vs_plain": (B / step_ms) / (1.0 / plain_ms),
-        "batched_equals_plain": bool(same),
+        "batched_equals_plain": not bad,
+        "batched_mismatched_rigs": len(bad),
+        "batched_compared": f"disp and multiview maps of all {B} rigs against their scene computed alone",
         "padded_cells_per_step": cells,
-        "pipeline_valu_frac": 97 * cells / (step_ms * 1e-3) / VALU_PEAK,
-        "roofline": roofline,
+        "padded_cells_per_s": cells / (step_ms * 1e-3),
+        "roofline": roofline_entry(kern, launches, cells, step_ms, B, args.workload),
         "kernel_ms": kern,
         "kernel_launches_per_step": launches,
     }
+    ctx.close()
+    del dev
+    torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(rigs[0])
-    ctx.close()
     return out
 
 
-def run_serve(args):
+def run_mixed(args):
+    """The mixed batch: 32 rigs (sizes 640x480 .. 2048x1536, D 64 .. 256) in
+    one captured graph."""
+    import torch
+    import sister_amd
+
+    ctx = sister_amd.Context(device=0)
+    rigs = make_rigs("mixed32", 32)
+    dev = to_device(ctx, rigs, torch)
+    evals = sum(r["W"] * r["H"] * r["D"] for r in rigs)
+    cells = sum(padded_cells(r["W"], r["H"], r["D"]) for r in rigs)
+    torch.cuda.synchronize()
+    ctx.graph_capture([d["struct"] for d in dev], 1)
+    for _ in range(args.warmup):
+        ctx.graph_launch()
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.graph_launch()
+    ctx.sync()
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ctx.close()
+    del dev
+    torch.cuda.empty_cache()
+    log(f"mixed32: {step_ms:.2f} ms per step")
+    return {"workload": "32 rigs, sizes uniform over {640x480, 1280x720, 1920x1080, 2048x1536}, D uniform over "
+                        "{64, 128, 192, 256} (seed 1000), one hipGraph",
+            "ms_per_step": step_ms, "rigs_per_s": 32 / (step_ms * 1e-3),
+            "value": evals / (step_ms * 1e-3) / 1e9, "unit": "G disparity-evals/s",
+            "padded_cells_per_s": cells / (step_ms * 1e-3), "steps": args.steps}
+
+
+def run_serve(args, seconds=None):
     """Continuous batching under Poisson arrivals: the mixed-size pool (640x480
     to 2048x1536, D 64..256) plus 5% 4096x3000 rigs, HBM budget --hbm-gb.
     Default: device-resident buffers (sister_submit_device); --serve-host
@@ -275,12 +444,13 @@ def run_serve(args):
     import sister_amd
     from sister_amd import synth
 
+    seconds = seconds or args.serve_seconds
     ctx = sister_amd.Context(device=0, hbm_budget=args.hbm_gb << 30)
     shapes = synth.batch_shapes(64, 1000)
     pool = [synth.rig_scene(W, H, D, 1000 + i) for i, (W, H, D) in enumerate(shapes[:8])]
     large = synth.rig_scene(4096, 3000, shapes[0][2], 5, n_boxes=20, blur_sigma=0.0)
     rng = np.random.default_rng(7)
-    n = int(args.rate * args.serve_seconds)
+    n = int(args.rate * seconds)
     # request i uses the large rig with probability 0.05, else a pool rig
     picks = [-1 if rng.random() < 0.05 else int(rng.integers(0, len(pool))) for _ in range(n)]
     gaps = rng.exponential(1.0 / args.rate, size=n)
@@ -322,23 +492,31 @@ def run_serve(args):
         except sister_amd.SisterError as e:
             failed += 1
             reason = str(e)
-        if time.perf_counter() - tlog > 2.0:
-            tlog = time.perf_counter()
-            log(f"serve: {i + 1}/{n} completed")
     wall = time.perf_counter() - t0
     st = ctx.stats()
     ctx.serve_stop()
     ctx.close()
+    slots.clear()
+    torch.cuda.empty_cache()
     evals = sum(rig_of(k)["W"] * rig_of(k)["H"] * rig_of(k)["D"] for k in picks)
+    log(f"serve: {n / wall:.1f} rigs/s, p50 {st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms")
     return {
-        "metric": METRIC, "value": evals / wall / 1e9, "unit": "G disparity-evals/s", "n_gpus": 1,
-        "steps": n, "warmup": 0, "ms_per_step": wall * 1e3 / n, "higher_is_better": True,
+        "workload": (f"serving: Poisson {args.rate:g} rigs/s for {seconds:g} s, mixed sizes/D + 5% 4096x3000, "
+                     f"{'device-resident' if device else 'host'} buffers, HBM budget {args.hbm_gb} GiB"),
+        "requests": n, "rigs_per_s": n / wall, "value": evals / wall / 1e9, "unit": "G disparity-evals/s",
+        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+        "large_rigs": sum(1 for k in picks if k < 0), "failed": failed, "fail_reason": reason,
+        "rejected": st["rejected"], "hbm_peak": st["hbm_peak"],
+    }
+
+
+def serve_line(args):
+    s = run_serve(args)
+    return {
+        "metric": METRIC, "value": s["value"], "unit": "G disparity-evals/s", "n_gpus": 1,
+        "steps": s["requests"], "warmup": 0, "ms_per_step": 1e3 / s["rigs_per_s"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
-        "config": {"workload": f"serving: Poisson {args.rate} rigs/s for {args.serve_seconds} s, mixed sizes/D "
-                               f"+ 5% 4096x3000, {'device-resident' if device else 'host'} buffers, "
-                               f"HBM budget {args.hbm_gb} GiB"},
-        "rigs_per_s": n / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-        "large_rigs": sum(1 for k in picks if k < 0), "failed": failed, "fail_reason": reason, "stats": st,
+        "config": {"workload": s["workload"]}, **s,
     }
 
 
@@ -369,6 +547,7 @@ def main():
     ap.add_argument("--batch", type=int, default=48)
     ap.add_argument("--workload", default="rig1080", choices=["rig1080", "mixed32", "large", "serve"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="headline only (no mixed32 / serve blocks)")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-host", action="store_true", help="serve host buffers instead of device-resident ones")
@@ -376,17 +555,23 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--single", action="store_true", help="one eager run of the batch (counter runs)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "bench.py measures one GPU only; --gpus must be 1 (rigs are independent, "
+                                   "a multi-GPU run would shard rigs across processes)"}))
+        sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
         return
     if args.single:
         run_single(args)
         return
-    if args.gpus != 1:
-        print(json.dumps({"error": "bench.py measures one GPU only; --gpus must be 1 (rigs are independent, "
-                                   "a multi-GPU run would shard rigs across processes)"}))
-        sys.exit(2)
-    out = run_serve(args) if args.workload == "serve" else run(args)
+    if args.workload == "serve":
+        print(json.dumps(serve_line(args)))
+        return
+    out = run(args)
+    if args.workload == "rig1080" and not args.no_extra:
+        out["mixed32"] = run_mixed(args)
+        out["serve"] = run_serve(args, seconds=5.0)
     print(json.dumps(out))
 
 
