@@ -2445,7 +2445,8 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
             TD(kKRows, 1);
             TE_S(kKRows, st);
             TB_S(kKVSweep, st);
-            TD(kKVSweep, launch_vsweep(G, mode, keep_sum, st));
+            const int nl = launch_vsweep(G, mode, keep_sum, st);
+            TD(kKVSweep, nl);
             TE_S(kKVSweep, st);
         }
         return;
@@ -2497,7 +2498,8 @@ void launch_sgm(const LaunchSet& ls, int mode, int keep_sum, hipStream_t st, con
             TD(kKRows, 1);
             TE_S(kKRows, st);
             TB_S(kKVSweep, st);
-            TD(kKVSweep, launch_vsweep(G, mode, keep_sum, st));
+            const int nl = launch_vsweep(G, mode, keep_sum, st);
+            TD(kKVSweep, nl);
             TE_S(kKVSweep, st);
             continue;
         }
