@@ -88,6 +88,20 @@ int sister_compute_rig(sister_ctx* ctx, const sister_rig* rig, int modes);
    any sizes run in one launch per stage on the context stream; asynchronous:
    call sister_sync() before reading outputs. */
 int sister_run_device(sister_ctx* ctx, const sister_rig* rigs, int n, int modes);
+
+/* sister_run_device without any device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   The rigs' workspaces and launch tables are carved from a fixed device array
+   of SISTER_NOALLOC_MAX_BYTES that each GPU holds from the library's load to
+   the end of the process (one 640x480 D=64 rig fits; 1920x1080 D=192 does
+   not).  Rigs that need more are refused with SISTER_ENOMEM before anything
+   is launched.  Every call on one GPU shares that GPU's array: calls that
+   overlap on one GPU (from two contexts, or two threads) race.  Calls on one
+   context are ordered: each waits for the context's previous work. */
+#define SISTER_NOALLOC_MAX_BYTES ((size_t)256 << 20)
+int sister_run_device_noalloc(sister_ctx* ctx, const sister_rig* rigs, int n, int modes);
+
 int sister_sync(sister_ctx* ctx);
 /* the hipStream_t the context launches on */
 void* sister_stream(sister_ctx* ctx);

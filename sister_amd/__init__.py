@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libsister_hip.so")
 
 SISTER_OK, SISTER_EINVAL, SISTER_ENOMEM, SISTER_EDEVICE, SISTER_EBUSY, SISTER_ENOTFOUND = range(6)
 MODE_MULTIVIEW, MODE_HORIZONTAL, MODE_VERTICAL, MODE_ALL = 1, 2, 4, 7
+NOALLOC_MAX_BYTES = 256 << 20   # SISTER_NOALLOC_MAX_BYTES
 
 _u8p = C.POINTER(C.c_uint8)
 _u16p = C.POINTER(C.c_uint16)
@@ -60,6 +61,7 @@ _SIGS = {
     "sister_last_error": (C.c_char_p, [C.c_void_p]),
     "sister_compute_rig": (C.c_int, [C.c_void_p, C.POINTER(RigStruct), C.c_int]),
     "sister_run_device": (C.c_int, [C.c_void_p, C.POINTER(RigStruct), C.c_int, C.c_int]),
+    "sister_run_device_noalloc": (C.c_int, [C.c_void_p, C.POINTER(RigStruct), C.c_int, C.c_int]),
     "sister_sync": (C.c_int, [C.c_void_p]),
     "sister_stream": (C.c_void_p, [C.c_void_p]),
     "sister_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
@@ -202,6 +204,12 @@ class Context:
     def run_device(self, rigs, modes=MODE_MULTIVIEW):
         arr = (RigStruct * len(rigs))(*rigs)
         self._check(lib().sister_run_device(self._h, arr, len(rigs), modes))
+
+    def run_device_noalloc(self, rigs, modes=MODE_MULTIVIEW):
+        """run_device with no device allocation: workspaces in the library's
+        fixed scratch array (NOALLOC_MAX_BYTES per GPU)."""
+        arr = (RigStruct * len(rigs))(*rigs)
+        self._check(lib().sister_run_device_noalloc(self._h, arr, len(rigs), modes))
 
     def sync(self):
         self._check(lib().sister_sync(self._h))

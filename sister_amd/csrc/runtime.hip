@@ -168,9 +168,53 @@ struct Plan {
     std::vector<Block> workspaces;
     Block table{nullptr, 0};
     LaunchSet ls{};
+    bool in_scratch = false;          // carved from the scratch array, not the arena
+};
+
+// Bump allocation over a fixed device range (the scratch array below).
+struct Carve {
+    char* base;
+    size_t size, used;
+    int get(size_t n, Block& b)
+    {
+        const size_t o = align_up(used);
+        if (o > size || n > size - o) return -1;
+        b = {base + o, n};
+        used = o + n;
+        return 0;
+    }
 };
 
 }  // namespace
+
+// Device memory of the allocation-free entry point (sister_run_device_noalloc):
+// the workspaces and launch tables of its rigs are carved from this array.
+// Each GPU the library runs on holds its own copy until the process ends.
+// In host code the name is only a stand-in for the symbol: the device address
+// comes from hipGetSymbolAddress, once per device (scratch_address below).
+constexpr size_t kScratchBytes = SISTER_NOALLOC_MAX_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on `device`, which must be current; nullptr if
+// the runtime cannot give it.
+static char* scratch_address(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_ptr[device] = (char*)p;
+    }
+    return g_scratch_ptr[device];
+}
 
 struct sister_ctx {
     int device = 0;
@@ -183,6 +227,8 @@ struct sister_ctx {
     bool keep_sum = false;
     int vsweep = -1;              // vertical SGM families: -1 automatic, 0 separate, 1 fused
     Plan last;                 // plan of the last sister_run_device (kept for debug fetch)
+    Plan noalloc;              // plan of the last sister_run_device_noalloc (in the scratch array)
+    char* scratch = nullptr;   // this device's scratch array (scratch_address)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     // a captured batch runs as independent parts on their own streams
@@ -287,10 +333,24 @@ static void record_latency(sister_ctx* c, double ms)
 // Build the device plan for n rigs whose views/outputs are device pointers.
 // The rig table is ordered by D so that the SGM launches can take each group
 // of equal depth as one contiguous slice.
-static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
+static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Carve* carve = nullptr)
 {
     p.host_rigs.clear();
     p.workspaces.clear();
+    p.in_scratch = carve != nullptr;
+    auto get = [&](size_t bytes, Block& b) { return carve ? carve->get(bytes, b) : c->pool.get(bytes, b); };
+    auto drop = [&]() {
+        if (!carve)
+            for (auto& w : p.workspaces) c->pool.put(w);
+        p.workspaces.clear();
+    };
+    auto no_room = [&](const std::string& what) {
+        drop();
+        if (carve)
+            return fail(c, SISTER_ENOMEM, what + " does not fit the " + std::to_string(kScratchBytes) +
+                                              "-byte scratch of the allocation-free entry point (one 640x480 D=64 rig fits)");
+        return fail(c, SISTER_ENOMEM, what + " exceeds the HBM budget");
+    };
     p.slot.assign(n, 0);
     for (int i = 0; i < n; i++) {
         const sister_rig& r = rigs[i];
@@ -349,11 +409,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
             const WsLayout l = ws_layout(g.W, g.H, g.D, c->keep_sum, G.vcols == 0);
             lay[k] = l;
             Block b;
-            if (c->pool.get(l.total, b) != 0) {
-                for (auto& w : p.workspaces) c->pool.put(w);
-                p.workspaces.clear();
-                return fail(c, SISTER_ENOMEM, "workspace of " + std::to_string(l.total) + " bytes exceeds the HBM budget");
-            }
+            if (get(l.total, b) != 0) return no_room("workspace of " + std::to_string(l.total) + " bytes");
             p.workspaces.push_back(b);
             const size_t npx = (size_t)g.Wp * g.Hp;
             for (int q = 0; q < 8; q++) {
@@ -372,8 +428,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
             g.sum = c->keep_sum ? (uint16_t*)(b.ptr + l.sum) : nullptr;
         }
     if ((int)groups.size() > kMaxDGroups) {
-        for (auto& w : p.workspaces) c->pool.put(w);
-        p.workspaces.clear();
+        drop();
         return fail(c, SISTER_EINVAL, "too many distinct disparity counts in one batch");
     }
     // table: rigs | scratch ptrs | bs[kNumStages][n+1] | per group kNumGroupKernels x (ng+1)
@@ -417,11 +472,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
         for (int f = 0; f < kNumGroupKernels; f++) gbs[gpos + f * (ng + 1) + ng] = tot[f];
         gpos += kNumGroupKernels * (size_t)(ng + 1);
     }
-    if (c->pool.get(host.size(), p.table) != 0) {
-        for (auto& w : p.workspaces) c->pool.put(w);
-        p.workspaces.clear();
-        return fail(c, SISTER_ENOMEM, "plan table allocation failed");
-    }
+    if (get(host.size(), p.table) != 0) return no_room("the launch tables of " + std::to_string(n) + " rigs");
     HIPCHK(c, hipMemcpy(p.table.ptr, host.data(), host.size(), hipMemcpyHostToDevice));
     p.ls.rigs = (const RigDesc*)p.table.ptr;
     p.ls.nrig = n;
@@ -463,11 +514,14 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
 
 static void free_plan(sister_ctx* c, Plan& p)
 {
-    for (auto& w : p.workspaces) c->pool.put(w);
+    if (!p.in_scratch) {
+        for (auto& w : p.workspaces) c->pool.put(w);
+        c->pool.put(p.table);
+    }
     p.workspaces.clear();
-    c->pool.put(p.table);
     p.table = {nullptr, 0};
     p.host_rigs.clear();
+    p.in_scratch = false;
 }
 
 // enqueue every stage of a plan on `st`
@@ -536,6 +590,8 @@ int sister_create(const sister_config* cfg, sister_ctx** out)
                  hipEventCreateWithFlags(&f.join[j], hipEventDisableTiming) == hipSuccess;
     }
     ok = ok && kernels_setup(c->device) == 0;
+    // a failed look-up leaves the allocation-free entry point reporting it
+    if (ok) c->scratch = scratch_address(c->device);
     if (!ok) {
         delete c;
         return SISTER_EDEVICE;
@@ -616,6 +672,25 @@ int sister_run_device(sister_ctx* c, const sister_rig* rigs, int n, int modes)
     if (rc) return rc;
     // profiling times every kernel on its own: the SGM families then run in sequence
     enqueue(c, c->last, modes, c->stream, c->profiling ? nullptr : &c->fam[0], c->profiling ? &c->timer : nullptr);
+    HIPCHK(c, hipGetLastError());
+    return SISTER_OK;
+}
+
+int sister_run_device_noalloc(sister_ctx* c, const sister_rig* rigs, int n, int modes)
+{
+    if (!c || !rigs || n <= 0) return fail(c, SISTER_EINVAL, "bad arguments");
+    int rc = check_modes(c, rigs, n, modes);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->scratch) return fail(c, SISTER_EDEVICE, "the scratch array's device address is unavailable");
+    // this context's previous call may still be reading the scratch array
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_plan(c, c->noalloc);
+    Carve carve{c->scratch, kScratchBytes, 0};
+    rc = build_plan(c, rigs, n, c->noalloc, &carve);
+    if (rc) return rc;
+    enqueue(c, c->noalloc, modes, c->stream, c->profiling ? nullptr : &c->fam[0], c->profiling ? &c->timer : nullptr);
     HIPCHK(c, hipGetLastError());
     return SISTER_OK;
 }
