@@ -539,6 +539,29 @@ def calibrate_copy(gib=1):
                       "ms_per_launch": ms, "copy_GBps": 2 * n * 4 / (ms * 1e-3) / 1e9}))
 
 
+def valu_peak():
+    """Sustained VALU issue rate of v_add_u32, v_pk_add_u16 and v_bcnt_u32_b32
+    (sister_amd/csrc/valu_peak.hip: 16 independent accumulators per lane, no
+    memory traffic), against the spec-sheet 256 CUs x 64 lanes x 2.4 GHz."""
+    import ctypes as C
+    import torch  # noqa: F401  (binds the HIP runtime torch uses)
+    path = os.path.join(ROOT, "sister_amd", "libvalu_peak.so")
+    lib = C.CDLL(path)
+    lib.valu_peak_run.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    out = {"spec_peak_lane_ops": VALU_PEAK}
+    for op, name, per_inst in ((0, "v_add_u32", 1), (1, "v_pk_add_u16", 2), (2, "v_bcnt_u32_b32", 1)):
+        ms, ops = C.c_double(), C.c_double()
+        rc = lib.valu_peak_run(op, 256 * 16, 8192, C.byref(ms), C.byref(ops))
+        if rc != 0:
+            raise RuntimeError(f"valu_peak_run({op}) returned {rc}")
+        rate = ops.value / (ms.value * 1e-3)
+        inst_rate = rate / per_inst
+        out[name] = {"ms": ms.value, "lane_ops_per_s": rate, "instr_lane_issues_per_s": inst_rate,
+                     "frac_of_spec_issue": inst_rate / VALU_PEAK,
+                     "implied_clock_GHz": inst_rate / (256 * 64) / 1e9}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -553,6 +576,7 @@ def main():
     ap.add_argument("--serve-host", action="store_true", help="serve host buffers instead of device-resident ones")
     ap.add_argument("--hbm-gb", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--valu-peak", action="store_true", help="VALU throughput probe (int32, packed u16, popcount)")
     ap.add_argument("--single", action="store_true", help="one eager run of the batch (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -561,6 +585,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.valu_peak:
+        valu_peak()
         return
     if args.single:
         run_single(args)

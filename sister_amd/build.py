@@ -48,7 +48,26 @@ def build(verbose=False):
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     build_cli(verbose)
+    build_probe(verbose)
     return LIB
+
+
+PROBE = os.path.join(HERE, "libvalu_peak.so")
+
+
+def build_probe(verbose=False):
+    """The VALU throughput probe (csrc/valu_peak.hip) as a library of its own:
+    bench.py --valu-peak loads it; the product library does not contain it."""
+    tmp = PROBE + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-shared", os.path.join(CSRC, "valu_peak.hip"), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed on valu_peak.hip")
+    os.replace(tmp, PROBE)
+    return PROBE
 
 
 CLI = os.path.join(HERE, "compute_disp")
