@@ -69,14 +69,14 @@ def test_normalize_minmax(colorize_check):
 
 
 def test_magma_lut(colorize_check):
-    lut = _run(colorize_check, "lut").reshape(256, 3)[:, ::-1]   # BGR -> RGB
-    # matplotlib's magma at 0, 127/255 and 1 is about (0.0015, 0.0005, 0.0139),
-    # (0.713, 0.214, 0.476) and (0.987, 0.991, 0.750); the polynomial fit stays
-    # within 6/255 of those points (parity with OpenCV's own table is unpinned)
-    for i, rgb in ((0, (0.001462, 0.000466, 0.013866)), (127, (0.712803, 0.213521, 0.475780)),
-                   (255, (0.987053, 0.991438, 0.749504))):
-        assert np.all(np.abs(lut[i] - np.round(np.array(rgb) * 255)) <= 6), (i, lut[i])
-    lum = lut @ np.array([0.299, 0.587, 0.114])
+    """The table equals matplotlib's listed magma data rounded to 8 bits as
+    OpenCV's convertTo does (tests/golden/make_magma.py), entry for entry."""
+    lut = _run(colorize_check, "lut").reshape(256, 3)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "magma_lut.npz"))
+    assert np.array_equal(lut, g["bgr8"].astype(lut.dtype))
+    # and against the float data directly: within half a level of 255 * v
+    assert np.all(np.abs(lut[:, ::-1] - 255.0 * g["rgb"]) <= 0.5 + 1e-4)
+    lum = lut[:, ::-1] @ np.array([0.299, 0.587, 0.114])
     assert np.all(np.diff(lum) >= -0.5)      # lightness rises along the map
     assert lum[255] - lum[0] > 200
 
