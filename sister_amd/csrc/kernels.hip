@@ -427,6 +427,18 @@ __global__ __launch_bounds__(64 * kWtaRowMaxWaves) void k_wta_row(const RigDesc*
 //   min(s4, max(s3, u0), max(s2, u1), max(s1, u2))
 // (k-th smallest of the union of two sorted lists; s0 and s5 can never be the
 // median).  The serial pass is then a few min/max per pixel.
+// compile-time loop: f(integral_constant<int, I>) for I in [B, E), so register
+// arrays indexed by I are never indexed at run time
+template <int I> struct IC { static constexpr int value = I; };
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(IC<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cswap(unsigned& a, unsigned& b)
 {
@@ -511,6 +523,17 @@ template <int K> struct MedBuf<K, false> { uint64_t v[K]; uint64_t l; };        
 // frames read the prepass's order statistics (fewer registers per slot).
 constexpr int kMedMaxK = 16;
 constexpr int kMedRawK = 4;
+// row buffers of the serial median's prefetch ring: the sorting form holds six
+// values per slot (three buffers fit its register budget, four spill), the
+// prepass form one (four fit); frames wider than the sorting form's keep two
+#ifndef SISTER_MED_RING_RAW
+#define SISTER_MED_RING_RAW 3
+#endif
+#ifndef SISTER_MED_RING_PRE
+#define SISTER_MED_RING_PRE 4
+#endif
+template <int K, bool RAW>
+constexpr int med_ring() { return RAW ? SISTER_MED_RING_RAW : K <= kMedRawK ? SISTER_MED_RING_PRE : 2; }
 
 template <int K, bool EXACT, bool PF, bool RAW>
 __device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t* __restrict__ in,
@@ -620,17 +643,26 @@ __device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t*
         uint16_t* t = prev; prev = cur; cur = t;
     };
     if constexpr (PF) {
-        MedBuf<K, RAW> bufA, bufB;
-        fetch(1, bufA);
+        // a ring of kMedRing buffers: row r's statistics are requested
+        // kMedRing - 1 rows ahead of their use, so that the row chain waits
+        // on load latency only when a row computes faster than that
+        constexpr int NB = med_ring<K, RAW>();
+        MedBuf<K, RAW> buf[NB];
+        static_for<0, NB - 1>([&](auto ic) { fetch(min(1 + decltype(ic)::value, R - 1), buf[decltype(ic)::value]); });
         __syncthreads();
         int r = 1;
-        for (; r + 1 < R; r += 2) {
-            fetch(r + 1, bufB);
-            row(r, bufA);
-            fetch(min(r + 2, R - 1), bufA);
-            row(r + 1, bufB);
+        for (; r + NB - 1 < R; r += NB) {
+            static_for<0, NB>([&](auto ic) __attribute__((always_inline)) {
+                constexpr int i = decltype(ic)::value;
+                fetch(min(r + i + NB - 1, R - 1), buf[(i + NB - 1) % NB]);
+                row(r + i, buf[i]);
+            });
         }
-        if (r < R) row(r, bufA);
+        // fewer than NB rows remain; all of them are fetched already
+        static_for<0, NB - 1>([&](auto ic) __attribute__((always_inline)) {
+            constexpr int i = decltype(ic)::value;
+            if (r + i < R) row(r + i, buf[i]);
+        });
     } else {
         MedBuf<K, RAW> buf;
         __syncthreads();
@@ -646,14 +678,23 @@ __device__ __forceinline__ void median_serial_body(int R, int C, const uint16_t*
 // from the prepass; MODE 2: every frame from the prepass, for plans with
 // frames wider than that (all maps at once: a rig's wide and narrow frames
 // take the same time, so they run side by side).
+// Blocks are dispatched long maps first: block b takes map slot b / nrig of rig
+// b % nrig, slots 0-3 being the rig's four maps of the frames with more rows
+// (the serial chain is one row per step) and slots 4-7 the other four.  When
+// the blocks take more than one round of the CUs, the short maps then fill
+// the CUs the long ones leave, instead of long maps starting last.
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restrict__ rigs,
+__global__ __launch_bounds__(1024) void k_median_serial(const RigDesc* __restrict__ rigs, int nrig,
                                                         uint64_t* const* __restrict__ scratch)
 {
     extern __shared__ uint16_t lds16[];
-    const int ri = blockIdx.x >> 3, m = blockIdx.x & 7;
+    const int ri = blockIdx.x % nrig, slot = blockIdx.x / nrig;
     const RigDesc& g = rigs[ri];
-    const int F = m >> 1;
+    // frames 1/3 have W' rows, frames 0/2 H' rows; map m = 2 F + side
+    const int long_odd = g.Wp > g.Hp ? 1 : 0;
+    const int fr = (slot >> 1) & 1, side = slot & 1;             // frame pair index, side
+    const int F = ((slot < 4) == (long_odd == 1)) ? 2 * fr + 1 : 2 * fr;
+    const int m = 2 * F + side;
     const int R = frame_rows(g, F), C = frame_cols(g, F);
     const uint16_t* in = g.wta[m];
     uint16_t* out = g.med[m];
@@ -752,18 +793,6 @@ __global__ __launch_bounds__(256) void k_mask(const RigDesc* __restrict__ rigs, 
 // LDS reads at immediate offsets and coalesced 64-byte stores per group, with
 // no per-disparity branches.  Cell stride Dp, zero past D.
 // ---------------------------------------------------------------------------
-// compile-time loop: f(integral_constant<int, I>) for I in [B, E), so register
-// arrays indexed by I are never indexed at run time
-template <int I> struct IC { static constexpr int value = I; };
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F&& f)
-{
-    if constexpr (B < E) {
-        f(IC<B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
 constexpr int kRun = 256;   // pixels per block
 constexpr int kWinPad = 8;  // window entries before each partner window
 
@@ -896,6 +925,9 @@ __global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ ri
 // Per pixel as above: one wave per pixel, lane q owns d = q + 64 j.
 // ---------------------------------------------------------------------------
 constexpr int kMT = 16;
+#ifndef SISTER_MSUM_HOIST
+#define SISTER_MSUM_HOIST 0
+#endif
 
 // window line stride (entries) for J groups of 64 disparities: holds
 // kWinPad + kMT + D - 1 <= 64 J + 23 entries; a compile-time constant, so a
@@ -1036,6 +1068,29 @@ __global__ __launch_bounds__(1024) void k_msum_tile(const RigDesc* __restrict__ 
 #pragma unroll
             for (int j = 0; j < J; j++) acc[j] = 0;
             const int mb = __builtin_amdgcn_readlane(mcol, ty);   // one pixel per wave
+#if SISTER_MSUM_HOIST
+            // every frame's window and centre reads are issued before any
+            // frame's (mask-dependent) sum, so one LDS latency covers the
+            // pixel instead of one per frame
+            uint64_t tv[4][J], cc[4];
+            static_for<0, 4>([&](auto kc) __attribute__((always_inline)) {
+                constexpr int F = decltype(kc)::value;
+                constexpr int off = F == 0 || F == 2 ? ty * LSTR : F == 1 ? kMT - 1 - ty : ty;
+#pragma unroll
+                for (int j = 0; j < J - 1; j++) tv[F][j] = pa[F][off + 64 * (J - 1 - j)];
+                tv[F][J - 1] = pl[F][off];
+                cc[F] = pc0[F * kMT * kMT + ty * kMT];
+            });
+            static_for<0, 4>([&](auto kc) __attribute__((always_inline)) {
+                constexpr int F = (0x3120 >> (4 * decltype(kc)::value)) & 15;   // frames 0, 2, 1, 3
+                if (!(mb & (1 << F))) return;
+#pragma unroll
+                for (int j = 0; j < J; j++) {
+                    const uint64_t xv = cc[F] ^ tv[F][j];
+                    acc[j] = bcnt_add((uint32_t)(xv >> 32), bcnt_add((uint32_t)xv, acc[j]));
+                }
+            });
+#else
             static_for<0, 4>([&](auto kc) __attribute__((always_inline)) {
                 constexpr int F = (0x3120 >> (4 * decltype(kc)::value)) & 15;   // frames 0, 2, 1, 3
                 constexpr int off = F == 0 || F == 2 ? ty * LSTR : F == 1 ? kMT - 1 - ty : ty;
@@ -1051,6 +1106,7 @@ __global__ __launch_bounds__(1024) void k_msum_tile(const RigDesc* __restrict__ 
                     acc[j] = bcnt_add((uint32_t)(xv >> 32), bcnt_add((uint32_t)xv, acc[j]));
                 }
             });
+#endif
             gptr<uint8_t> op = o + ty * rstride;
 #pragma unroll
             for (int j = 0; j < J - 1; j++) op[q + 64 * j] = (uint8_t)acc[j];
@@ -2229,9 +2285,9 @@ void launch_median(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
         TE(kKMedianPrep);
     }
     TB(kKMedianSerial);
-    if (raw) hipLaunchKernelGGL(k_median_serial<0>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
-    else if (wide) hipLaunchKernelGGL(k_median_serial<2>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
-    else hipLaunchKernelGGL(k_median_serial<1>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.scratch);
+    if (raw) hipLaunchKernelGGL(k_median_serial<0>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.nrig, ls.scratch);
+    else if (wide) hipLaunchKernelGGL(k_median_serial<2>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.nrig, ls.scratch);
+    else hipLaunchKernelGGL(k_median_serial<1>, dim3(ls.nrig * 8), dim3(1024), lds, st, ls.rigs, ls.nrig, ls.scratch);
     TD(kKMedianSerial, 1);
     TE(kKMedianSerial);
 }
