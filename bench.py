@@ -288,6 +288,20 @@ def roofline_entry(kern, launches, cells, step_ms, B, workload):
     return roof
 
 
+def time_graph(ctx, args, torch):
+    """Replays of the context's captured graph: warm-up, then the timed steps."""
+    for _ in range(args.warmup):
+        ctx.graph_launch()
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.graph_launch()
+    ctx.sync()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / args.steps
+
+
 def run(args):
     import numpy as np
     import torch
@@ -337,21 +351,24 @@ def run(args):
     ctx.sync()
 
     # ---- batched: all B rigs in one captured plan ----------------------------
+    # first in the default launch forms, then in the forms the autotuner picks
+    # for this batch (graph parts; per disparity group the SGM and cost-sum
+    # forms), which is what is reported
     for d in dev:
         d["outs"]["disp"].zero_()
         d["outs"]["multiview"].zero_()
-    ctx.graph_capture([d["struct"] for d in dev], 1)
-    for _ in range(args.warmup):
-        ctx.graph_launch()
-    ctx.sync()
-    torch.cuda.synchronize()
+    structs = [d["struct"] for d in dev]
+    ctx.graph_capture(structs, 1)
+    default_ms = time_graph(ctx, args, torch)
+    log(f"batched, default forms: {default_ms:.2f} ms per step")
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.graph_launch()
-    ctx.sync()
-    torch.cuda.synchronize()
-    step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    log(f"batched: {step_ms:.2f} ms per step")
+    ctx.autotune(structs, 1)
+    tune_s = time.perf_counter() - t0
+    for d in dev:
+        d["outs"]["disp"].zero_()
+        d["outs"]["multiview"].zero_()
+    step_ms = time_graph(ctx, args, torch)
+    log(f"batched, tuned forms: {step_ms:.2f} ms per step (tuning {tune_s:.1f} s)")
     bad = [i for i in range(B)
            if not all(torch.equal(dev[i]["outs"][k], plain_out[i % nscene][k]) for k in ("disp", "multiview"))]
 
@@ -382,6 +399,9 @@ def run(args):
         "plain": {"ms_per_rig": plain_ms, "rigs_per_s": 1e3 / plain_ms,
                   "value": plain_evals / (plain_ms * 1e-3) / 1e9},
         "batched_vs_plain": (B / step_ms) / (1.0 / plain_ms),
+        "default_forms_ms_per_step": default_ms,
+        "autotune": {"seconds": tune_s, "forms": ctx.tune_table().strip().splitlines()[1:],
+                     "note": "kernel_ms comes from eager steps in the default forms"},
         "batched_equals_plain": not bad,
         "batched_mismatched_rigs": len(bad),
         "batched_compared": f"disp and multiview maps of all {B} rigs against their scene computed alone",
@@ -412,26 +432,35 @@ def run_mixed(args):
     evals = sum(r["W"] * r["H"] * r["D"] for r in rigs)
     cells = sum(padded_cells(r["W"], r["H"], r["D"]) for r in rigs)
     torch.cuda.synchronize()
-    ctx.graph_capture([d["struct"] for d in dev], 1)
-    for _ in range(args.warmup):
-        ctx.graph_launch()
-    ctx.sync()
-    torch.cuda.synchronize()
+    structs = [d["struct"] for d in dev]
+    # each rig alone, for the comparison below
+    alone = []
+    for d in dev:
+        ctx.run_device([d["struct"]], 1)
+        ctx.sync()
+        alone.append(d["outs"]["disp"].clone())
+    ctx.graph_capture(structs, 1)
+    default_ms = time_graph(ctx, args, torch)
+    log(f"mixed32, default forms: {default_ms:.2f} ms per step")
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.graph_launch()
-    ctx.sync()
-    torch.cuda.synchronize()
-    step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ctx.autotune(structs, 1)
+    tune_s = time.perf_counter() - t0
+    for d in dev:
+        d["outs"]["disp"].zero_()
+    step_ms = time_graph(ctx, args, torch)
+    bad = sum(1 for d, a in zip(dev, alone) if not torch.equal(d["outs"]["disp"], a))
+    forms = ctx.tune_table().strip().splitlines()[1:]
     ctx.close()
     del dev
     torch.cuda.empty_cache()
-    log(f"mixed32: {step_ms:.2f} ms per step")
+    log(f"mixed32, tuned forms: {step_ms:.2f} ms per step (tuning {tune_s:.1f} s)")
     return {"workload": "32 rigs, sizes uniform over {640x480, 1280x720, 1920x1080, 2048x1536}, D uniform over "
                         "{64, 128, 192, 256} (seed 1000), one hipGraph",
             "ms_per_step": step_ms, "rigs_per_s": 32 / (step_ms * 1e-3),
             "value": evals / (step_ms * 1e-3) / 1e9, "unit": "G disparity-evals/s",
-            "padded_cells_per_s": cells / (step_ms * 1e-3), "steps": args.steps}
+            "padded_cells_per_s": cells / (step_ms * 1e-3), "steps": args.steps,
+            "default_forms_ms_per_step": default_ms, "autotune_seconds": tune_s, "forms": forms,
+            "rigs_differing_from_alone": bad}
 
 
 def run_serve(args, seconds=None):
@@ -565,7 +594,7 @@ def valu_peak():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=48)
     ap.add_argument("--workload", default="rig1080", choices=["rig1080", "mixed32", "large", "serve"])

@@ -117,6 +117,26 @@ int sister_outside_reference_domain(int disp_count);
 int sister_graph_capture(sister_ctx* ctx, const sister_rig* rigs, int n, int modes);
 int sister_graph_launch(sister_ctx* ctx);
 
+/* ---- autotuning ----------------------------------------------------------- */
+/* No counterpart in the reference (its launch shapes are fixed by its SSE
+   loops).  Times on the device, as a captured graph of this batch, the launch
+   forms that leave results unchanged: first the graph's parts (1 to 4 parts
+   balanced by padded cells, or one part per disparity count, so that groups
+   of different D run side by side), then for each disparity group of those
+   parts the SGM vertical families fused or separate, the lanes per chain of
+   the separate families, and the tiled or two-launch cost sum.  The fastest
+   of each is kept in the context's table, which every later plan and capture
+   of a matching shape consults: groups are matched by D and the half-octaves
+   of their padded column total and tallest padded rig, batches by rig count,
+   distinct D count, half-octave of padded cells and modes.  Shapes already in
+   the table are not timed again.  path (nullable): a forms file, read first
+   and rewritten with the table.  The batch is left captured in its tuned
+   form (sister_graph_launch replays it). */
+int sister_autotune(sister_ctx* ctx, const sister_rig* rigs, int n, int modes, const char* path);
+/* The context's table of tuned forms as text, one form per line (the forms
+   file's format). */
+int sister_tune_describe(sister_ctx* ctx, char* buf, size_t len);
+
 /* ---- serving: continuous batching by HBM footprint ---------------------- */
 typedef struct {
     uint64_t submitted, completed, rejected;

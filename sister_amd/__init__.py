@@ -68,6 +68,8 @@ _SIGS = {
     "sister_outside_reference_domain": (C.c_int, [C.c_int]),
     "sister_graph_capture": (C.c_int, [C.c_void_p, C.POINTER(RigStruct), C.c_int, C.c_int]),
     "sister_graph_launch": (C.c_int, [C.c_void_p]),
+    "sister_autotune": (C.c_int, [C.c_void_p, C.POINTER(RigStruct), C.c_int, C.c_int, C.c_char_p]),
+    "sister_tune_describe": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "sister_serve_start": (C.c_int, [C.c_void_p, C.c_int]),
     "sister_submit": (C.c_int, [C.c_void_p, C.POINTER(RigStruct), C.c_int, C.POINTER(C.c_uint64)]),
     "sister_submit_device": (C.c_int, [C.c_void_p, C.POINTER(RigStruct), C.c_int, C.POINTER(C.c_uint64)]),
@@ -220,6 +222,19 @@ class Context:
 
     def graph_launch(self):
         self._check(lib().sister_graph_launch(self._h))
+
+    def autotune(self, rigs, modes=MODE_MULTIVIEW, path=None):
+        """Time the launch forms of this batch (graph parts, then each D
+        group's SGM and cost-sum forms) and keep the fastest; the batch is
+        left captured in its tuned form.  path: forms file read and rewritten."""
+        arr = (RigStruct * len(rigs))(*rigs)
+        self._check(lib().sister_autotune(self._h, arr, len(rigs), modes,
+                                          path.encode() if path else None))
+
+    def tune_table(self):
+        buf = C.create_string_buffer(1 << 16)
+        self._check(lib().sister_tune_describe(self._h, buf, len(buf)))
+        return buf.value.decode()
 
     # -- serving -------------------------------------------------------------
     def serve_start(self, max_inflight=4):

@@ -32,6 +32,7 @@ struct DGroup {
     int CH[4], KL[4];        // SGM: chunks of 16 disparities per lane, lanes per chain, per family
     int vcols;               // fused vertical sweep: owned columns per block (0: separate families)
     int maxHp;               // tallest padded rig of the group
+    int msum_split;          // 1: the two-launch cost sum even where the tiled one fits
 };
 constexpr int kMaxDGroups = 16;
 

@@ -2402,7 +2402,7 @@ void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t)
     TB(kKMsum);
     for (int gi = 0; gi < ls.ngroups; gi++) {
         const DGroup& G = ls.groups[gi];
-        if ((fuse & 5) && (fuse & 10) && msum_tile_fits(G) && !msum_split_forced()) {
+        if ((fuse & 5) && (fuse & 10) && msum_tile_fits(G) && !G.msum_split && !msum_split_forced()) {
             launch_msum_tile(G, fuse, st);                 // all frames, M written once
             TD(kKMsum, 1);
             continue;

@@ -195,6 +195,43 @@ struct Carve {
 constexpr size_t kScratchBytes = SISTER_NOALLOC_MAX_BYTES;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
+// ---------------------------------------------------------------------------
+// Launch forms chosen by the autotuner (sister_autotune).  A disparity group
+// of a plan (its rigs of one D) is keyed by D and the octaves of its padded
+// column total and tallest padded rig; a captured batch by its rig count, its
+// distinct D count, the octave of its padded cells and its modes.  Every form
+// gives the same outputs; only the time differs.
+// ---------------------------------------------------------------------------
+struct GroupForm {
+    int fused = -1;           // -1 automatic, 0 separate chain families, 1 fused vertical sweep
+    int kl[4] = {0, 0, 0, 0}; // lanes per chain of the separate families (0: automatic)
+    int msum_split = 0;       // 1: the two-launch cost sum
+    bool operator==(const GroupForm& o) const
+    {
+        return fused == o.fused && msum_split == o.msum_split && kl[0] == o.kl[0] && kl[1] == o.kl[1] &&
+               kl[2] == o.kl[2] && kl[3] == o.kl[3];
+    }
+};
+struct BatchForm {
+    int parts = 0;            // independent graph parts (0: automatic)
+    int by_d = 0;             // 1: one part per disparity count
+};
+static int octave(double v) { return v <= 1 ? 0 : (int)std::floor(std::log2(v) * 2.0); }   // half octaves
+struct GroupKey {
+    int D, cols_b, hp_b;
+    bool operator<(const GroupKey& o) const
+    {
+        return D != o.D ? D < o.D : cols_b != o.cols_b ? cols_b < o.cols_b : hp_b < o.hp_b;
+    }
+};
+struct BatchKey {
+    int n, nd, cells_b, modes;
+    bool operator<(const BatchKey& o) const
+    {
+        return n != o.n ? n < o.n : nd != o.nd ? nd < o.nd : cells_b != o.cells_b ? cells_b < o.cells_b : modes < o.modes;
+    }
+};
+
 constexpr int kMaxDevices = 64;
 static std::mutex g_scratch_mu;
 static char* g_scratch_ptr[kMaxDevices] = {};
@@ -228,6 +265,8 @@ struct sister_ctx {
     int vsweep = -1;              // vertical SGM families: -1 automatic, 0 separate, 1 fused
     Plan last;                 // plan of the last sister_run_device (kept for debug fetch)
     Plan noalloc;              // plan of the last sister_run_device_noalloc (in the scratch array)
+    std::map<GroupKey, GroupForm> gforms;   // autotuned group forms (under mu)
+    std::map<BatchKey, BatchForm> bforms;   // autotuned batch forms (under mu)
     char* scratch = nullptr;   // this device's scratch array (scratch_address)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -384,19 +423,27 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Car
         p.host_rigs.push_back(g);
     }
     // SGM groups: runs of equal D
-    struct GroupHost { int i0, i1, KL[4], CH[4], vcols, maxHp; };
+    struct GroupHost { int i0, i1, KL[4], CH[4], vcols, maxHp, msum_split; };
     std::vector<GroupHost> groups;
     for (int k = 0; k < n;) {
         int e = k;
         long chains = 0;
         while (e < n && p.host_rigs[e].D == p.host_rigs[k].D) chains += p.host_rigs[e++].Wp;
-        GroupHost G{k, e, {}, {}, 0, 0};
-        for (int f = 0; f < 4; f++) {
-            G.KL[f] = sgm_pick_kl(chains, f, p.host_rigs[k].D);
-            G.CH[f] = sgm_pick_ch(p.host_rigs[k].D, G.KL[f]);
-        }
-        G.vcols = vsweep_pick_cols(chains, p.host_rigs[k].D, c->vsweep);
+        GroupHost G{k, e, {}, {}, 0, 0, 0};
         for (int j = k; j < e; j++) G.maxHp = std::max(G.maxHp, p.host_rigs[j].Hp);
+        const int D = p.host_rigs[k].D;
+        GroupForm form;
+        const auto it = c->gforms.find(GroupKey{D, octave((double)chains), octave((double)G.maxHp)});
+        if (it != c->gforms.end()) form = it->second;
+        for (int f = 0; f < 4; f++) {
+            const int kl = form.kl[f];
+            const bool ok = (kl == 8 || kl == 16 || (kl == 4 && D <= 256));
+            G.KL[f] = ok ? kl : sgm_pick_kl(chains, f, D);
+            G.CH[f] = sgm_pick_ch(D, G.KL[f]);
+        }
+        // the debug switch wins over a tuned form
+        G.vcols = vsweep_pick_cols(chains, D, c->vsweep >= 0 ? c->vsweep : form.fused);
+        G.msum_split = form.msum_split;
         groups.push_back(G);
         k = e;
     }
@@ -508,6 +555,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Car
         }
         sg.vcols = G.vcols;
         sg.maxHp = G.maxHp;
+        sg.msum_split = G.msum_split;
     }
     return SISTER_OK;
 }
@@ -758,6 +806,71 @@ int sister_compute_rig(sister_ctx* c, const sister_rig* r, int modes)
     return rc;
 }
 
+}  // extern "C"
+
+static BatchKey batch_key(const sister_rig* rigs, int n, int modes)
+{
+    double cells = 0;
+    std::vector<int> ds;
+    for (int i = 0; i < n; i++) {
+        const sister_rig& r = rigs[i];
+        cells += (double)(r.width + 2 * r.disp_count) * (r.height + 2 * r.disp_count) * r.disp_count;
+        ds.push_back(r.disp_count);
+    }
+    std::sort(ds.begin(), ds.end());
+    const int nd = (int)(std::unique(ds.begin(), ds.end()) - ds.begin());
+    return BatchKey{n, nd, octave(cells), modes};
+}
+
+// The independent parts of a captured batch (parallel branches of its graph),
+// under mu.  By default (or as tuned) the rigs are balanced by padded cells
+// over the parts, largest first to the lightest part; the tuned form may
+// instead give each disparity count a part of its own, so that groups of
+// different D run side by side.  SISTER_GRAPH_PARTS (1..8) overrides the
+// default part count (comparison runs).
+static int partition_batch(sister_ctx* c, const sister_rig* rigs, int n, int modes,
+                           std::vector<sister_rig> (&part)[sister_ctx::kMaxParts])
+{
+    static const int env_parts = [] {
+        const char* e = getenv("SISTER_GRAPH_PARTS");
+        const int v = e ? atoi(e) : 0;
+        return v < 0 ? 0 : (v > sister_ctx::kMaxParts ? sister_ctx::kMaxParts : v);
+    }();
+    BatchForm form;
+    const auto it = c->bforms.find(batch_key(rigs, n, modes));
+    if (it != c->bforms.end()) form = it->second;
+    for (auto& p : part) p.clear();
+    if (form.by_d) {
+        std::vector<int> ds;
+        for (int i = 0; i < n; i++) ds.push_back(rigs[i].disp_count);
+        std::sort(ds.begin(), ds.end());
+        ds.erase(std::unique(ds.begin(), ds.end()), ds.end());
+        if ((int)ds.size() <= sister_ctx::kMaxParts) {
+            for (int i = 0; i < n; i++)
+                part[std::lower_bound(ds.begin(), ds.end(), rigs[i].disp_count) - ds.begin()].push_back(rigs[i]);
+            return (int)ds.size();
+        }
+    }
+    const int want = env_parts ? env_parts : form.parts > 0 ? form.parts : 2;
+    const int nparts = std::max(1, std::min(std::min(want, (int)sister_ctx::kMaxParts), n));
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    auto cells = [&](int i) {
+        const sister_rig& r = rigs[i];
+        return (double)(r.width + 2 * r.disp_count) * (r.height + 2 * r.disp_count) * r.disp_count;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    double load[sister_ctx::kMaxParts] = {};
+    for (int i : order) {
+        const int h = (int)(std::min_element(load, load + nparts) - load);
+        part[h].push_back(rigs[i]);
+        load[h] += cells(i);
+    }
+    return nparts;
+}
+
+extern "C" {
+
 int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes)
 {
     if (!c || !rigs || n <= 0) return fail(c, SISTER_EINVAL, "bad arguments");
@@ -769,29 +882,8 @@ int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
     for (auto& gp : c->graph_plan) free_plan(c, gp);
-    // SISTER_GRAPH_PARTS (1..8, default 2) sets how many independent parts a
-    // captured batch is split into (for tuning)
-    static const int want_parts = [] {
-        const char* e = getenv("SISTER_GRAPH_PARTS");
-        const int v = e ? atoi(e) : 2;
-        return v < 1 ? 1 : (v > sister_ctx::kMaxParts ? sister_ctx::kMaxParts : v);
-    }();
-    const int nparts = std::min(want_parts, n);
-    // parts balanced by padded cells (largest rig first to the lightest part)
-    std::vector<int> order(n);
-    for (int i = 0; i < n; i++) order[i] = i;
-    auto cells = [&](int i) {
-        const sister_rig& r = rigs[i];
-        return (double)(r.width + 2 * r.disp_count) * (r.height + 2 * r.disp_count) * r.disp_count;
-    };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
     std::vector<sister_rig> part[sister_ctx::kMaxParts];
-    double load[sister_ctx::kMaxParts] = {};
-    for (int i : order) {
-        const int h = (int)(std::min_element(load, load + nparts) - load);
-        part[h].push_back(rigs[i]);
-        load[h] += cells(i);
-    }
+    const int nparts = partition_batch(c, rigs, n, modes, part);
     for (int h = 0; h < nparts; h++) {
         rc = build_plan(c, part[h].data(), (int)part[h].size(), c->graph_plan[h]);
         if (rc) {
@@ -821,6 +913,197 @@ int sister_graph_launch(sister_ctx* c)
     if (!c || !c->graph_exec) return fail(c, SISTER_EINVAL, "no captured graph");
     HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
     return SISTER_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// autotuning
+// ---------------------------------------------------------------------------
+static std::string forms_text(const sister_ctx* c)
+{
+    std::string t = "# sister launch forms v1\n";
+    char line[256];
+    for (const auto& kv : c->bforms) {
+        std::snprintf(line, sizeof line, "batch %d %d %d %d parts %d by_d %d\n", kv.first.n, kv.first.nd,
+                      kv.first.cells_b, kv.first.modes, kv.second.parts, kv.second.by_d);
+        t += line;
+    }
+    for (const auto& kv : c->gforms) {
+        const GroupForm& f = kv.second;
+        std::snprintf(line, sizeof line, "group %d %d %d fused %d kl %d %d %d %d msum_split %d\n", kv.first.D,
+                      kv.first.cols_b, kv.first.hp_b, f.fused, f.kl[0], f.kl[1], f.kl[2], f.kl[3], f.msum_split);
+        t += line;
+    }
+    return t;
+}
+
+// Entries of a forms file into the tables (existing keys are replaced).
+// Returns false on a malformed line.
+static bool forms_parse(sister_ctx* c, FILE* fp)
+{
+    char line[512];
+    while (std::fgets(line, sizeof line, fp)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        BatchKey bk;
+        BatchForm bf;
+        GroupKey gk;
+        GroupForm gf;
+        if (std::sscanf(line, "batch %d %d %d %d parts %d by_d %d", &bk.n, &bk.nd, &bk.cells_b, &bk.modes,
+                        &bf.parts, &bf.by_d) == 6) {
+            c->bforms[bk] = bf;
+        } else if (std::sscanf(line, "group %d %d %d fused %d kl %d %d %d %d msum_split %d", &gk.D, &gk.cols_b,
+                               &gk.hp_b, &gf.fused, &gf.kl[0], &gf.kl[1], &gf.kl[2], &gf.kl[3],
+                               &gf.msum_split) == 9) {
+            c->gforms[gk] = gf;
+        } else {
+            return false;
+        }
+    }
+    return true;
+}
+
+// Group keys of the plans a capture of this batch would build (under mu).
+static std::vector<GroupKey> batch_group_keys(sister_ctx* c, const sister_rig* rigs, int n, int modes)
+{
+    std::vector<sister_rig> part[sister_ctx::kMaxParts];
+    const int np = partition_batch(c, rigs, n, modes, part);
+    std::vector<GroupKey> keys;
+    for (int h = 0; h < np; h++) {
+        std::map<int, std::pair<long, int>> g;   // D -> (columns, tallest)
+        for (const sister_rig& r : part[h]) {
+            auto& e = g[r.disp_count];
+            e.first += r.width + 2 * r.disp_count;
+            e.second = std::max(e.second, r.height + 2 * r.disp_count);
+        }
+        for (const auto& kv : g) {
+            const GroupKey k{kv.first, octave((double)kv.second.first), octave((double)kv.second.second)};
+            if (std::find_if(keys.begin(), keys.end(), [&](const GroupKey& o) { return !(o < k) && !(k < o); }) ==
+                keys.end())
+                keys.push_back(k);
+        }
+    }
+    return keys;
+}
+
+extern "C" {
+
+int sister_autotune(sister_ctx* c, const sister_rig* rigs, int n, int modes, const char* path)
+{
+    if (!c || !rigs || n <= 0) return fail(c, SISTER_EINVAL, "bad arguments");
+    int rc = check_modes(c, rigs, n, modes);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (path) {
+            FILE* fp = std::fopen(path, "r");
+            if (fp) {
+                const bool ok = forms_parse(c, fp);
+                std::fclose(fp);
+                if (!ok) return fail(c, SISTER_EINVAL, std::string("malformed launch-forms file ") + path);
+            }
+        }
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        free_plan(c, c->last);   // room for the captures
+    }
+    // one candidate: capture, one warm replay, then the mean of three
+    auto time_graph = [&](double& ms) -> int {
+        int r = sister_graph_capture(c, rigs, n, modes);
+        if (r) return r;
+        HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < 3; i++) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / 3;
+        return SISTER_OK;
+    };
+    const BatchKey bk = batch_key(rigs, n, modes);
+    bool tuned_batch;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        tuned_batch = c->bforms.count(bk) != 0;
+    }
+    // 1. the batch's graph parts: 1 to 4 balanced parts, or a part per D
+    if (!tuned_batch) {
+        std::vector<BatchForm> cand;
+        for (int p = 1; p <= std::min(4, n); p++) cand.push_back(BatchForm{p, 0});
+        if (bk.nd >= 2 && bk.nd <= sister_ctx::kMaxParts) cand.push_back(BatchForm{0, 1});
+        double best = 1e30;
+        BatchForm bestf;
+        for (const BatchForm& f : cand) {
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                c->bforms[bk] = f;
+            }
+            double ms = 0;
+            if ((rc = time_graph(ms))) return rc;
+            if (ms < best) { best = ms; bestf = f; }
+        }
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->bforms[bk] = bestf;
+    }
+    // 2. each disparity group of those parts: vertical families fused or
+    // separate, lanes per chain, cost-sum form (the other groups keep theirs)
+    std::vector<GroupKey> keys;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        keys = batch_group_keys(c, rigs, n, modes);
+    }
+    for (const GroupKey& k : keys) {
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            if (c->gforms.count(k)) continue;
+        }
+        std::vector<GroupForm> cand;
+        GroupForm f;
+        cand.push_back(f);                                       // automatic
+        f.fused = 0; cand.push_back(f);                          // separate families, automatic lanes
+        if (k.D <= 192) { f.fused = 1; cand.push_back(f); }      // fused vertical sweep
+        f = GroupForm(); f.fused = 0;
+        for (int q = 0; q < 4; q++) f.kl[q] = 16;
+        cand.push_back(f);                                       // separate, 16 lanes everywhere
+        f.kl[2] = f.kl[3] = 8; cand.push_back(f);                // separate, diagonals on 8
+        f = GroupForm(); f.msum_split = 1; cand.push_back(f);    // two-launch cost sum
+        double best = 1e30;
+        GroupForm bestf;
+        for (const GroupForm& g : cand) {
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                c->gforms[k] = g;
+            }
+            double ms = 0;
+            if ((rc = time_graph(ms))) return rc;
+            if (ms < best) { best = ms; bestf = g; }
+        }
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->gforms[k] = bestf;
+    }
+    // leave the batch captured in its tuned form
+    if ((rc = sister_graph_capture(c, rigs, n, modes))) return rc;
+    if (path) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        const std::string t = forms_text(c);
+        FILE* fp = std::fopen(path, "w");
+        if (!fp || std::fwrite(t.data(), 1, t.size(), fp) != t.size()) {
+            if (fp) std::fclose(fp);
+            return fail(c, SISTER_EINVAL, std::string("cannot write ") + path);
+        }
+        std::fclose(fp);
+    }
+    return SISTER_OK;
+}
+
+int sister_tune_describe(sister_ctx* c, char* buf, size_t len)
+{
+    if (!c || !buf || len == 0) return SISTER_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const std::string t = forms_text(c);
+    const size_t k = std::min(len - 1, t.size());
+    std::memcpy(buf, t.data(), k);
+    buf[k] = 0;
+    return t.size() < len ? SISTER_OK : fail(c, SISTER_EINVAL, "buffer too small for the forms table");
 }
 
 int sister_profile_enable(sister_ctx* c, int on)
