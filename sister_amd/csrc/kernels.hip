@@ -40,6 +40,16 @@ __device__ __forceinline__ void ST16(void* p, uint4 v) { *(gptr<u32x4>)p = u32x4
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
+// Block index b of an n-block grid taken in XCD-contiguous order: with
+// round-robin dispatch block b runs on XCD b % 8, so the XCD gets the
+// consecutive logical blocks [start(b % 8), start(b % 8 + 1)) (the first
+// n % 8 XCDs one more than the others).  A bijection of [0, n).
+__device__ __forceinline__ int xcd_order(int b, int n)
+{
+    const int x = b & 7, k = b >> 3, q = n >> 3, r = n & 7;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 __device__ __forceinline__ int find_rig(const int* __restrict__ bs, int n, int b)
 {
     int lo = 0, hi = n;
@@ -413,6 +423,18 @@ __global__ __launch_bounds__(64 * kWtaRowMaxWaves) void k_wta_row(const RigDesc*
     for (int p = threadIdx.x; p < C; p += blockDim.x) G(outR)[p] = (uint16_t)(RP[p] & 511u);
 }
 
+// compile-time loop: f(integral_constant<int, I>) for I in [B, E), so register
+// arrays indexed by I are never indexed at run time
+template <int I> struct IC { static constexpr int value = I; };
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(IC<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // K3 in-place 3x3 median (postprocess.cpp:15-71 called with source == dest).
 // Over flat index q of a R x C map, processed in increasing q:
@@ -427,18 +449,6 @@ __global__ __launch_bounds__(64 * kWtaRowMaxWaves) void k_wta_row(const RigDesc*
 //   min(s4, max(s3, u0), max(s2, u1), max(s1, u2))
 // (k-th smallest of the union of two sorted lists; s0 and s5 can never be the
 // median).  The serial pass is then a few min/max per pixel.
-// compile-time loop: f(integral_constant<int, I>) for I in [B, E), so register
-// arrays indexed by I are never indexed at run time
-template <int I> struct IC { static constexpr int value = I; };
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F&& f)
-{
-    if constexpr (B < E) {
-        f(IC<B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cswap(unsigned& a, unsigned& b)
 {
@@ -1694,6 +1704,9 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 // ([parity][cols, diag, anti][W'][Dp + 16] bytes: the normalised (L - M)
 // bytes, then minL').
 // ---------------------------------------------------------------------------
+#ifndef SISTER_VS_XCD
+#define SISTER_VS_XCD 1
+#endif
 constexpr int kVGhost = 16;        // ghost columns per side = rows per launch
 constexpr int kVCols = 144;        // owned columns per block (16 blocks per 2304-wide rig)
 constexpr int kVKL = 4;            // lanes per column
@@ -1761,7 +1774,15 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     constexpr bool DEFER = !KEEP;
     constexpr int KL = kVKL, NP = 8 * CH;
     extern __shared__ uint32_t xs[];               // FL slots, then FR slots
-    const int ri = find_rig(bs, nrig, blockIdx.x);
+    // neighbouring column blocks on one XCD: a block's ghost columns are its
+    // neighbours' owned ones, whose cost bytes and states then come from the
+    // same L2
+#if SISTER_VS_XCD
+    const int bid = xcd_order(blockIdx.x, gridDim.x);
+#else
+    const int bid = blockIdx.x;
+#endif
+    const int ri = find_rig(bs, nrig, bid);
     const RigDesc& g = rigs[ri];
     const int Wp = g.Wp, Hp = g.Hp, D = g.D, Dp = g.Dp;
     const int t0 = seg * kVGhost;
@@ -1771,7 +1792,7 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     constexpr int nt = kVCols + 2 * kVGhost;
     const int tid = threadIdx.x, c = tid / KL;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = (int)(blockDim.x >> 6);
-    const int x = (blockIdx.x - bs[ri]) * kVCols - kVGhost + c;
+    const int x = (bid - bs[ri]) * kVCols - kVGhost + c;
     const bool act = x >= 0 && x < Wp;
     // every column of the wave inside the image: no border guards to apply
     const bool wave_act = __builtin_amdgcn_ballot_w64(!act) == 0;
