@@ -1708,12 +1708,14 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 #define SISTER_VS_XCD 1
 #endif
 constexpr int kVGhost = 16;        // ghost columns per side = rows per launch
-constexpr int kVCols = 144;        // owned columns per block (16 blocks per 2304-wide rig)
-constexpr int kVKL = 4;            // lanes per column
 constexpr int kVMaxThreads = 768;  // 12 waves: up to 168 VGPRs per lane
+// Lanes per column: 4 up to D = 192 (3 chunks of 16 per lane), 8 up to D =
+// 384.  Owned columns per block: 144 with 4 lanes (176 computed, 11 waves;
+// 16 blocks per 2304-wide rig), 64 with 8 (96 computed, 12 waves).
+__host__ __device__ constexpr int vs_cols(int KL) { return KL == 4 ? 144 : 64; }
 // 8-byte granules per pair slot of the family state slots: nt computed
 // columns plus the two border columns, KL lanes each, rounded up to 64
-__host__ __device__ constexpr int vs_gs(int nt) { return ((nt + 2) * kVKL + 63) / 64 * 64; }
+__host__ __device__ constexpr int vs_gs(int nt, int KL) { return ((nt + 2) * KL + 63) / 64 * 64; }
 
 template <int CH>
 __device__ __forceinline__ void vs_guard_all(LaneState<CH>& st)
@@ -1758,12 +1760,12 @@ __device__ __forceinline__ uint32_t vs_key(uint32_t pair, uint32_t m512)
     return r;
 }
 
-template <int CH, int DIR, bool KEEP, bool FULL>
+template <int CH, int DIR, bool KEEP, bool FULL, int VKL>
 __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restrict__ rigs, int nrig,
                                                         const int* __restrict__ bs, int ncol, int seg, int mode)
 {
     // KEEP: also the debug sums S (all rows) and the edge rows they need.
-    // FULL: D == Dp == 64 CH, every chunk of every lane is real (no padding
+    // FULL: D == Dp == 16 VKL CH, every chunk of every lane is real (no padding
     // chunk, no half chunk), so no per-chunk lane conditions are needed.
     constexpr bool keep_sum = KEEP;
     // deferred normalisation: within a segment the states stay raw (the step
@@ -1772,7 +1774,8 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     // Raw values grow by at most 255 per row, so 16 rows stay far below the
     // guard 0x3FFF.  The debug variant keeps exact states for its edge rows.
     constexpr bool DEFER = !KEEP;
-    constexpr int KL = kVKL, NP = 8 * CH;
+    constexpr int KL = VKL, NP = 8 * CH;
+    constexpr int VC = vs_cols(KL);                // owned columns per block
     extern __shared__ uint32_t xs[];               // FL slots, then FR slots
     // neighbouring column blocks on one XCD: a block's ghost columns are its
     // neighbours' owned ones, whose cost bytes and states then come from the
@@ -1788,18 +1791,21 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     const int t0 = seg * kVGhost;
     if (t0 >= Hp) return;                          // the rig has no such segment: whole block
     const int nrows = min(kVGhost, Hp - t0);
-    (void)ncol;                                    // kVCols (compile-time LDS strides)
-    constexpr int nt = kVCols + 2 * kVGhost;
+    (void)ncol;                                    // VC (compile-time LDS strides)
+    constexpr int nt = VC + 2 * kVGhost;
     const int tid = threadIdx.x, c = tid / KL;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = (int)(blockDim.x >> 6);
-    const int x = (bid - bs[ri]) * kVCols - kVGhost + c;
+    const int x = (bid - bs[ri]) * VC - kVGhost + c;
     const bool act = x >= 0 && x < Wp;
     // every column of the wave inside the image: no border guards to apply
     const bool wave_act = __builtin_amdgcn_ballot_w64(!act) == 0;
     // ghost waves step all three families too (their outward-flowing chains
     // and columns are never read by an owned column), so no wave branches
-    // around a step; the busiest SIMD still carries three owned waves
-    const bool own = wave != 0 && wave != nwaves - 1;
+    // around a step; the busiest SIMD still carries three owned waves.  The
+    // kVGhost ghost columns of a side fill kVGhost KL / 64 waves.
+    constexpr int GW = kVGhost * KL / 64;
+    static_assert(GW * 64 == kVGhost * KL, "ghost columns fill whole waves");
+    const bool own = wave >= GW && wave < nwaves - GW;
     ChainGeo cg;
     cg.lid = tid % KL;
     cg.nq = FULL ? KL * CH : Dp >> 4;
@@ -1812,7 +1818,7 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     // minL' = 0), written once and never overwritten, so reads need no test.
     // GS is a multiple of 64 granules, so that the compiler pairs the slot
     // accesses (ds_read2st64 / ds_write2st64).
-    constexpr int GS = vs_gs(nt);
+    constexpr int GS = vs_gs(nt, KL);
     uint32_t* XL = xs;
     uint32_t* XR = xs + NP * GS + nt + 2;
     constexpr int famL = DIR == 0 ? 2 : 3, famR = DIR == 0 ? 3 : 2;   // family ids of FL, FR
@@ -2351,16 +2357,19 @@ int kernels_setup(int device)
             return 0;
         }
     g_lds_per_block = (size_t)lim;
-#define SISTER_VS_FNS(K, F)                                                                         \
-    (const void*)k_vsweep<1, 0, K, F>, (const void*)k_vsweep<2, 0, K, F>, (const void*)k_vsweep<3, 0, K, F>, \
-        (const void*)k_vsweep<1, 1, K, F>, (const void*)k_vsweep<2, 1, K, F>, (const void*)k_vsweep<3, 1, K, F>
+#define SISTER_VS_FNS(K, F)                                                         \
+    (const void*)k_vsweep<1, 0, K, F, 4>, (const void*)k_vsweep<2, 0, K, F, 4>,     \
+        (const void*)k_vsweep<3, 0, K, F, 4>, (const void*)k_vsweep<1, 1, K, F, 4>, \
+        (const void*)k_vsweep<2, 1, K, F, 4>, (const void*)k_vsweep<3, 1, K, F, 4>, \
+        (const void*)k_vsweep<2, 0, K, F, 8>, (const void*)k_vsweep<3, 0, K, F, 8>, \
+        (const void*)k_vsweep<2, 1, K, F, 8>, (const void*)k_vsweep<3, 1, K, F, 8>
     for (const void* f : {(const void*)k_wta_row<false>, (const void*)k_wta_row<true>})
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
             (void)hipGetLastError();
             g_lds_per_block = 0;
             return 0;
         }
-    const void* vfns[18] = {SISTER_VS_FNS(false, false), SISTER_VS_FNS(false, true), SISTER_VS_FNS(true, false)};
+    const void* vfns[30] = {SISTER_VS_FNS(false, false), SISTER_VS_FNS(false, true), SISTER_VS_FNS(true, false)};
 #undef SISTER_VS_FNS
     for (const void* f : vfns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
@@ -2435,12 +2444,19 @@ void launch_msum(const LaunchSet& ls, int fuse, hipStream_t st, KernelTimer* t)
     TE(kKMsum);
 }
 
-static int vs_ch(int D) { return ((D + 15) / 16 + kVKL - 1) / kVKL; }
+// lanes per column of the fused sweep for D (0: beyond its reach, D > 384)
+static int vs_kl(int D)
+{
+    const int k16 = (D + 15) / 16;
+    return (k16 + 3) / 4 <= 3 ? 4 : (k16 + 7) / 8 <= 3 ? 8 : 0;
+}
+static int vs_ch(int D) { const int kl = vs_kl(D); return kl ? ((D + 15) / 16 + kl - 1) / kl : 0; }
 
 static size_t vs_lds(int D, int vcols)
 {
+    const int kl = vs_kl(D);
     const size_t nt = vcols + 2 * kVGhost;
-    return 2 * ((size_t)8 * vs_ch(D) * vs_gs((int)nt) + nt + 2) * sizeof(uint32_t);
+    return 2 * ((size_t)8 * vs_ch(D) * vs_gs((int)nt, kl) + nt + 2) * sizeof(uint32_t);
 }
 
 int vsweep_pick_cols(long cols, int D, int choice)
@@ -2452,10 +2468,10 @@ int vsweep_pick_cols(long cols, int D, int choice)
         return e ? atoi(e) : -1;
     }();
     const int forced = choice >= 0 ? choice : env;
-    // up to 192 disparities (3 chunks per lane): beyond, the state spills
-    if (forced == 0 || vs_ch(D) > 3) return 0;
-    // 144 owned columns: 176 computed, 11 waves; 16 blocks per 2304-wide rig
-    const int vcols = kVCols;
+    // up to 3 chunks of 16 per lane (beyond, the state spills): 4 lanes per
+    // column up to D = 192, 8 up to D = 384
+    if (forced == 0 || vs_kl(D) == 0) return 0;
+    const int vcols = vs_cols(vs_kl(D));
     if (vs_lds(D, vcols) > g_lds_per_block) return 0;
     // enough blocks to fill the chip (few rigs keep the per-chain families,
     // which spread a single rig over every CU)
@@ -2470,38 +2486,51 @@ int vsweep_blocks(const RigDesc& g, int vcols)
 
 static int launch_vsweep(const DGroup& G, int mode, int keep_sum, hipStream_t st)
 {
+    const int kl = vs_kl(G.D);
     const int nt = G.vcols + 2 * kVGhost;
     const size_t lds = vs_lds(G.D, G.vcols);
-    const dim3 grid(G.blocks[kGVSweep]), block(kVKL * nt);
+    const dim3 grid(G.blocks[kGVSweep]), block(kl * nt);
     const int* bs = G.bs[kGVSweep];
     const int nseg = (G.maxHp + kVGhost - 1) / kVGhost;
-    const int ch = vs_ch(G.D);   // 1..3: vsweep_pick_cols gives no columns beyond 3 chunks
-    const bool full = G.D == G.Dp && G.Dp == 64 * ch;
+    // 1..3 chunks with 4 lanes per column, 2..3 with 8 (vsweep_pick_cols
+    // gives no columns beyond)
+    const int ch = vs_ch(G.D);
+    const bool full = G.D == G.Dp && G.Dp == 16 * kl * ch;
     for (int dir = 0; dir < 2; dir++)
         for (int seg = 0; seg < nseg; seg++) {
-#define SISTER_VS(CHV, DIRV)                                                                                \
-    do {                                                                                                    \
-        if (keep_sum)                                                                                       \
-            hipLaunchKernelGGL((k_vsweep<CHV, DIRV, true, false>), grid, block, lds, st, G.rigs, G.nrig, bs, \
-                               G.vcols, seg, mode);                                                         \
-        else if (full)                                                                                      \
-            hipLaunchKernelGGL((k_vsweep<CHV, DIRV, false, true>), grid, block, lds, st, G.rigs, G.nrig, bs, \
-                               G.vcols, seg, mode);                                                         \
-        else                                                                                                \
-            hipLaunchKernelGGL((k_vsweep<CHV, DIRV, false, false>), grid, block, lds, st, G.rigs, G.nrig,   \
-                               bs, G.vcols, seg, mode);                                                     \
+#define SISTER_VS(CHV, DIRV, KLV)                                                                              \
+    do {                                                                                                       \
+        if (keep_sum)                                                                                          \
+            hipLaunchKernelGGL((k_vsweep<CHV, DIRV, true, false, KLV>), grid, block, lds, st, G.rigs, G.nrig,  \
+                               bs, G.vcols, seg, mode);                                                        \
+        else if (full)                                                                                         \
+            hipLaunchKernelGGL((k_vsweep<CHV, DIRV, false, true, KLV>), grid, block, lds, st, G.rigs, G.nrig,  \
+                               bs, G.vcols, seg, mode);                                                        \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_vsweep<CHV, DIRV, false, false, KLV>), grid, block, lds, st, G.rigs, G.nrig, \
+                               bs, G.vcols, seg, mode);                                                        \
     } while (0)
-            if (dir == 0) {
-                switch (ch) {
-                case 1: SISTER_VS(1, 0); break;
-                case 2: SISTER_VS(2, 0); break;
-                default: SISTER_VS(3, 0); break;
+            if (kl == 4) {
+                if (dir == 0) {
+                    switch (ch) {
+                    case 1: SISTER_VS(1, 0, 4); break;
+                    case 2: SISTER_VS(2, 0, 4); break;
+                    default: SISTER_VS(3, 0, 4); break;
+                    }
+                } else {
+                    switch (ch) {
+                    case 1: SISTER_VS(1, 1, 4); break;
+                    case 2: SISTER_VS(2, 1, 4); break;
+                    default: SISTER_VS(3, 1, 4); break;
+                    }
                 }
             } else {
-                switch (ch) {
-                case 1: SISTER_VS(1, 1); break;
-                case 2: SISTER_VS(2, 1); break;
-                default: SISTER_VS(3, 1); break;
+                if (dir == 0) {
+                    if (ch == 2) SISTER_VS(2, 0, 8);
+                    else SISTER_VS(3, 0, 8);
+                } else {
+                    if (ch == 2) SISTER_VS(2, 1, 8);
+                    else SISTER_VS(3, 1, 8);
                 }
             }
 #undef SISTER_VS

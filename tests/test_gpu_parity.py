@@ -125,6 +125,35 @@ def test_fused_vertical_full_lanes_outputs(gpu_ctx, oracle_mod, W, H, D, seed):
     assert float(np.abs(got["subpix"] - ref["subpix"]).max()) <= 1e-3
 
 
+# the fused sweep's 8-lane form (D in (192, 384]: 64 owned columns, 2 or 3
+# chunks per lane): every stage (debug sums kept) and the outputs of its
+# full-lane and partial forms, one and several blocks
+@pytest.mark.parametrize("W,H,D,seed", [(16, 16, 200, 31), (16, 20, 256, 32), (16, 16, 264, 33)])
+def test_stages_bit_exact_fused_vertical_8_lanes(gpu_ctx, oracle_mod, W, H, D, seed):
+    print(f"[parity fused 8 lanes] W={W} H={H} D={D} seed={seed}", flush=True)
+    gpu_ctx.vsweep(1)
+    try:
+        bad = _compare_stages(gpu_ctx, oracle_mod, W, H, D, seed)
+    finally:
+        gpu_ctx.vsweep(-1)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("W,H,D,seed", [(16, 16, 256, 34), (16, 16, 384, 35), (24, 16, 320, 36), (200, 16, 256, 37)])
+def test_fused_vertical_8_lanes_outputs(gpu_ctx, oracle_mod, W, H, D, seed):
+    print(f"[parity fused 8 lanes, outputs] W={W} H={H} D={D} seed={seed}", flush=True)
+    rig = synth.rig_small(W, H, D, seed)
+    ref = oracle_mod.compute(rig["views"], D, modes=1)
+    gpu_ctx.vsweep(1)
+    try:
+        got = gpu_ctx.compute_rig(rig["views"], D, modes=1)
+    finally:
+        gpu_ctx.vsweep(-1)
+    for k in ("multiview", "disp"):
+        assert np.array_equal(got[k], ref[k]), (k, int((got[k] != ref[k]).sum()))
+    assert float(np.abs(got["subpix"] - ref["subpix"]).max()) <= 1e-3
+
+
 def test_plain_640x480_d64(gpu_ctx, oracle_mod):
     print("[parity] 640x480 D=64 seed=1", flush=True)
     rig = synth.rig_plain_640()
