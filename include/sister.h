@@ -122,7 +122,8 @@ int sister_graph_launch(sister_ctx* ctx);
    loops).  Times on the device, as a captured graph of this batch, the launch
    forms that leave results unchanged: first the graph's parts (1 to 4 parts
    balanced by padded cells, or one part per disparity count, so that groups
-   of different D run side by side), then for each disparity group of those
+   of different D run side by side, and the parts started together or each
+   after the previous part's first stages), then for each disparity group of those
    parts the SGM vertical families fused or separate, the lanes per chain of
    the separate families, and the tiled or two-launch cost sum.  The fastest
    of each is kept in the context's table, which every later plan and capture

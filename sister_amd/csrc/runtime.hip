@@ -215,6 +215,7 @@ struct GroupForm {
 struct BatchForm {
     int parts = 0;            // independent graph parts (0: automatic)
     int by_d = 0;             // 1: one part per disparity count
+    int stagger = 0;          // > 0: part h + 1 starts once part h's stages [0, stagger) are done
 };
 static int octave(double v) { return v <= 1 ? 0 : (int)std::floor(std::log2(v) * 2.0); }   // half octaves
 struct GroupKey {
@@ -276,7 +277,7 @@ struct sister_ctx {
     static constexpr int kMaxParts = 8;
     Plan graph_plan[kMaxParts];
     hipStream_t side[kMaxParts] = {};
-    hipEvent_t ev_fork = nullptr, ev_join[kMaxParts] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxParts] = {}, ev_stage[kMaxParts] = {};
     FamStreams fam[kMaxParts] = {};   // per part: streams for concurrent SGM families
     std::mutex mu;             // guards pool / plans between the serving thread and callers
 
@@ -573,23 +574,36 @@ static void free_plan(sister_ctx* c, Plan& p)
 }
 
 // enqueue every stage of a plan on `st`
-static void enqueue(sister_ctx* c, const Plan& p, int modes, hipStream_t st, const FamStreams* fs, KernelTimer* t)
+// Stages of a plan, in order: 0 census, 1 WTA, 2 median, 3 masks, then per
+// mode m 4 + 2m the cost sum and 5 + 2m the SGM (with the winner).
+constexpr int kNumPlanStages = 10;
+
+// enqueue stages [from, to) of a plan on `st`
+static void enqueue_range(sister_ctx* c, const Plan& p, int modes, hipStream_t st, const FamStreams* fs,
+                          KernelTimer* t, int from, int to)
 {
     const LaunchSet& ls = p.ls;
-    launch_census(ls, st, t);
-    launch_wta(ls, st, t);
-    launch_median(ls, st, t);
+    auto in = [&](int k) { return k >= from && k < to; };
+    if (in(0)) launch_census(ls, st, t);
+    if (in(1)) launch_wta(ls, st, t);
+    if (in(2)) launch_median(ls, st, t);
     int fmask = 0;
     if (modes & SISTER_MODE_MULTIVIEW) fmask |= 0xF;
     if (modes & SISTER_MODE_HORIZONTAL) fmask |= 0x5;
     if (modes & SISTER_MODE_VERTICAL) fmask |= 0xA;
-    launch_mask(ls, fmask, st, t);
+    if (in(3)) launch_mask(ls, fmask, st, t);
     static const int fuse[3] = {0xF, 0x5, 0xA};   // hpp:264-275: frames summed per mode
     for (int m = 0; m < 3; m++) {
         if (!(modes & (1 << m))) continue;
-        launch_msum(ls, fuse[m], st, t);
-        launch_sgm(ls, m, c->keep_sum ? 1 : 0, st, fs, t);
+        if (in(4 + 2 * m)) launch_msum(ls, fuse[m], st, t);
+        if (in(5 + 2 * m)) launch_sgm(ls, m, c->keep_sum ? 1 : 0, st, fs, t);
     }
+}
+
+// enqueue every stage of a plan on `st`
+static void enqueue(sister_ctx* c, const Plan& p, int modes, hipStream_t st, const FamStreams* fs, KernelTimer* t)
+{
+    enqueue_range(c, p, modes, st, fs, t, 0, kNumPlanStages);
 }
 
 static int check_modes(sister_ctx* c, const sister_rig* rigs, int n, int modes)
@@ -630,6 +644,8 @@ int sister_create(const sister_config* cfg, sister_ctx** out)
     for (int k = 1; ok && k < sister_ctx::kMaxParts; k++)
         ok = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < sister_ctx::kMaxParts; k++)
+        ok = hipEventCreateWithFlags(&c->ev_stage[k], hipEventDisableTiming) == hipSuccess;
     for (int k = 0; ok && k < sister_ctx::kMaxParts; k++) {
         FamStreams& f = c->fam[k];
         ok = hipEventCreateWithFlags(&f.fork, hipEventDisableTiming) == hipSuccess;
@@ -671,6 +687,7 @@ void sister_destroy(sister_ctx* c)
     (void)hipEventDestroy(c->ev_fork);
     for (int k = 1; k < sister_ctx::kMaxParts; k++) {
         (void)hipEventDestroy(c->ev_join[k]);
+        (void)hipEventDestroy(c->ev_stage[k]);
         (void)hipStreamDestroy(c->side[k]);
     }
     for (auto& f : c->fam) {
@@ -829,16 +846,23 @@ static BatchKey batch_key(const sister_rig* rigs, int n, int modes)
 // different D run side by side.  SISTER_GRAPH_PARTS (1..8) overrides the
 // default part count (comparison runs).
 static int partition_batch(sister_ctx* c, const sister_rig* rigs, int n, int modes,
-                           std::vector<sister_rig> (&part)[sister_ctx::kMaxParts])
+                           std::vector<sister_rig> (&part)[sister_ctx::kMaxParts], int* stagger = nullptr)
 {
     static const int env_parts = [] {
         const char* e = getenv("SISTER_GRAPH_PARTS");
         const int v = e ? atoi(e) : 0;
         return v < 0 ? 0 : (v > sister_ctx::kMaxParts ? sister_ctx::kMaxParts : v);
     }();
+    // SISTER_GRAPH_STAGGER (1..9) overrides the stagger of the parts (comparison runs)
+    static const int env_stagger = [] {
+        const char* e = getenv("SISTER_GRAPH_STAGGER");
+        const int v = e ? atoi(e) : -1;
+        return v < 0 || v >= kNumPlanStages ? -1 : v;
+    }();
     BatchForm form;
     const auto it = c->bforms.find(batch_key(rigs, n, modes));
     if (it != c->bforms.end()) form = it->second;
+    if (stagger) *stagger = env_stagger >= 0 ? env_stagger : form.stagger;
     for (auto& p : part) p.clear();
     if (form.by_d) {
         std::vector<int> ds;
@@ -883,7 +907,8 @@ int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
     for (auto& gp : c->graph_plan) free_plan(c, gp);
     std::vector<sister_rig> part[sister_ctx::kMaxParts];
-    const int nparts = partition_batch(c, rigs, n, modes, part);
+    int stagger = 0;
+    const int nparts = partition_batch(c, rigs, n, modes, part, &stagger);
     for (int h = 0; h < nparts; h++) {
         rc = build_plan(c, part[h].data(), (int)part[h].size(), c->graph_plan[h]);
         if (rc) {
@@ -897,7 +922,20 @@ int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes
     // the SGM families of a part run in sequence inside a captured graph
     // (concurrent families are used by eager runs only: a capture that forked
     // them from the part streams ended in a host-side crash in the runtime)
-    for (int h = 0; h < nparts; h++) enqueue(c, c->graph_plan[h], modes, h ? c->side[h] : c->stream, nullptr, nullptr);
+    auto part_stream = [&](int h) { return h ? c->side[h] : c->stream; };
+    if (stagger > 0 && nparts > 1) {
+        // part h + 1's first stages wait for part h's: one part's SGM then
+        // runs beside the next part's census, WTA and cost sum
+        for (int h = 0; h < nparts; h++) {
+            if (h) (void)hipStreamWaitEvent(part_stream(h), c->ev_stage[h - 1], 0);
+            enqueue_range(c, c->graph_plan[h], modes, part_stream(h), nullptr, nullptr, 0, stagger);
+            (void)hipEventRecord(c->ev_stage[h], part_stream(h));
+        }
+        for (int h = 0; h < nparts; h++)
+            enqueue_range(c, c->graph_plan[h], modes, part_stream(h), nullptr, nullptr, stagger, kNumPlanStages);
+    } else {
+        for (int h = 0; h < nparts; h++) enqueue(c, c->graph_plan[h], modes, part_stream(h), nullptr, nullptr);
+    }
     for (int h = 1; h < nparts; h++) {
         (void)hipEventRecord(c->ev_join[h], c->side[h]);
         (void)hipStreamWaitEvent(c->stream, c->ev_join[h], 0);
@@ -925,8 +963,9 @@ static std::string forms_text(const sister_ctx* c)
     std::string t = "# sister launch forms v1\n";
     char line[256];
     for (const auto& kv : c->bforms) {
-        std::snprintf(line, sizeof line, "batch %d %d %d %d parts %d by_d %d\n", kv.first.n, kv.first.nd,
-                      kv.first.cells_b, kv.first.modes, kv.second.parts, kv.second.by_d);
+        std::snprintf(line, sizeof line, "batch %d %d %d %d parts %d by_d %d stagger %d\n", kv.first.n,
+                      kv.first.nd, kv.first.cells_b, kv.first.modes, kv.second.parts, kv.second.by_d,
+                      kv.second.stagger);
         t += line;
     }
     for (const auto& kv : c->gforms) {
@@ -949,8 +988,8 @@ static bool forms_parse(sister_ctx* c, FILE* fp)
         BatchForm bf;
         GroupKey gk;
         GroupForm gf;
-        if (std::sscanf(line, "batch %d %d %d %d parts %d by_d %d", &bk.n, &bk.nd, &bk.cells_b, &bk.modes,
-                        &bf.parts, &bf.by_d) == 6) {
+        if (std::sscanf(line, "batch %d %d %d %d parts %d by_d %d stagger %d", &bk.n, &bk.nd, &bk.cells_b,
+                        &bk.modes, &bf.parts, &bf.by_d, &bf.stagger) >= 6) {
             c->bforms[bk] = bf;
         } else if (std::sscanf(line, "group %d %d %d fused %d kl %d %d %d %d msum_split %d", &gk.D, &gk.cols_b,
                                &gk.hp_b, &gf.fused, &gf.kl[0], &gf.kl[1], &gf.kl[2], &gf.kl[3],
@@ -1040,6 +1079,21 @@ int sister_autotune(sister_ctx* c, const sister_rig* rigs, int n, int modes, con
             double ms = 0;
             if ((rc = time_graph(ms))) return rc;
             if (ms < best) { best = ms; bestf = f; }
+        }
+        // staggered starts of the chosen parts: after the masks, or after the
+        // first cost sum
+        if (bestf.parts != 1 || bestf.by_d) {
+            for (int st : {4, 5}) {
+                BatchForm f = bestf;
+                f.stagger = st;
+                {
+                    std::lock_guard<std::mutex> lk(c->mu);
+                    c->bforms[bk] = f;
+                }
+                double ms = 0;
+                if ((rc = time_graph(ms))) return rc;
+                if (ms < best) { best = ms; bestf = f; }
+            }
         }
         std::lock_guard<std::mutex> lk(c->mu);
         c->bforms[bk] = bestf;
