@@ -6,7 +6,7 @@ mode 0 (multiview).  Two forms are measured on the same synthetic rigs, with
 inputs and outputs resident in HBM before the timed region:
 
   plain    one rig per call of the library (sister_run_device with n = 1),
-  batched  B rigs in one plan (default 48: 182 GB of workspace), captured
+  batched  B rigs in one plan (default 56: 213 GB of workspace), captured
            once as a hipGraph and replayed.
 
 `value` is the batched throughput in G disparity-evals/s (sum of W*H*D over
@@ -78,8 +78,10 @@ OPS_PER_CELL = {"wta": 28, "msum": 4, "sgm_rows": 16, "sgm_cols": 16, "sgm_diag"
 # separate families).  The cost sum over 16x16 tiles writes M once (1); its
 # two-launch form (frames 0/180, then 90/270 read-modify-write) moves 3.
 BYTES_PER_CELL = {"sgm_rows": 4, "sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "sgm_vsweep": 8, "msum": 1}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch48_rig1080_d192.json")
-VALU_FILE = os.path.join(ROOT, "profiles", "valu_batch48_rig1080_d192.json")
+# per-kernel counter totals of one instrumented step (used when their batch
+# size is the one measured)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_rig1080_d192.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_rig1080_d192.json")
 
 
 def log(*a):
@@ -602,7 +604,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=48)
+    ap.add_argument("--batch", type=int, default=56)
     ap.add_argument("--workload", default="rig1080", choices=["rig1080", "mixed32", "large", "serve"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline only (no mixed32 / serve blocks)")

@@ -75,7 +75,10 @@ struct Pool {
         bytes = align_up(bytes);
         auto it = by_size.lower_bound(bytes);
         if (it == by_size.end()) {
-            size_t want = std::max(bytes, chunk);
+            // requests of half a chunk or more get a chunk of their own size
+            // (rounding a 3.8 GB rig workspace up to 4 GiB would leave a
+            // remainder per rig that large requests cannot use)
+            size_t want = bytes >= chunk / 2 ? bytes : std::max(bytes, chunk);
             if (held + want > budget) want = bytes;
             if (held + want > budget) trim();
             if (held + want > budget) return -1;

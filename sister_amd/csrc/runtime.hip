@@ -561,6 +561,18 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Car
     return SISTER_OK;
 }
 
+// build_plan for the batch entry points: when the arena's free memory is
+// split into chunks too small for the plan (earlier plans of other shapes),
+// its entirely free chunks go back to the device and the plan is built again
+// once.  (hipFree synchronises the device; this runs only on that failure.)
+static int build_plan_compact(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
+{
+    int rc = build_plan(c, rigs, n, p);
+    if (rc != SISTER_ENOMEM) return rc;
+    c->pool.trim();
+    return build_plan(c, rigs, n, p);
+}
+
 static void free_plan(sister_ctx* c, Plan& p)
 {
     if (!p.in_scratch) {
@@ -733,7 +745,7 @@ int sister_run_device(sister_ctx* c, const sister_rig* rigs, int n, int modes)
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));   // the previous plan may still be in use
     free_plan(c, c->last);
-    rc = build_plan(c, rigs, n, c->last);
+    rc = build_plan_compact(c, rigs, n, c->last);
     if (rc) return rc;
     // profiling times every kernel on its own: the SGM families then run in sequence
     enqueue(c, c->last, modes, c->stream, c->profiling ? nullptr : &c->fam[0], c->profiling ? &c->timer : nullptr);
@@ -909,13 +921,24 @@ int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes
     std::vector<sister_rig> part[sister_ctx::kMaxParts];
     int stagger = 0;
     const int nparts = partition_batch(c, rigs, n, modes, part, &stagger);
-    for (int h = 0; h < nparts; h++) {
-        rc = build_plan(c, part[h].data(), (int)part[h].size(), c->graph_plan[h]);
-        if (rc) {
-            for (auto& gp : c->graph_plan) free_plan(c, gp);
-            return rc;
+    auto build_parts = [&]() {
+        for (int h = 0; h < nparts; h++) {
+            const int r = build_plan(c, part[h].data(), (int)part[h].size(), c->graph_plan[h]);
+            if (r) {
+                for (auto& gp : c->graph_plan) free_plan(c, gp);
+                return r;
+            }
         }
+        return (int)SISTER_OK;
+    };
+    rc = build_parts();
+    if (rc == SISTER_ENOMEM) {
+        // the arena's free memory may sit in chunks shaped for earlier plans:
+        // return the entirely free ones to the device and build again once
+        c->pool.trim();
+        rc = build_parts();
     }
+    if (rc) return rc;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     (void)hipEventRecord(c->ev_fork, c->stream);
     for (int h = 1; h < nparts; h++) (void)hipStreamWaitEvent(c->side[h], c->ev_fork, 0);
@@ -1046,9 +1069,14 @@ int sister_autotune(sister_ctx* c, const sister_rig* rigs, int n, int modes, con
         HIPCHK(c, hipStreamSynchronize(c->stream));
         free_plan(c, c->last);   // room for the captures
     }
-    // one candidate: capture, one warm replay, then the mean of three
+    // one candidate: capture, one warm replay, then the mean of three; a form
+    // whose workspaces do not fit the budget is skipped (infinite time)
     auto time_graph = [&](double& ms) -> int {
         int r = sister_graph_capture(c, rigs, n, modes);
+        if (r == SISTER_ENOMEM) {
+            ms = 1e30;
+            return SISTER_OK;
+        }
         if (r) return r;
         HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
