@@ -6,7 +6,7 @@ mode 0 (multiview).  Two forms are measured on the same synthetic rigs, with
 inputs and outputs resident in HBM before the timed region:
 
   plain    one rig per call of the library (sister_run_device with n = 1),
-  batched  B rigs in one plan (default 56: 213 GB of workspace), captured
+  batched  B rigs in one plan (default 64: 243 GB of workspace), captured
            once as a hipGraph and replayed.
 
 `value` is the batched throughput in G disparity-evals/s (sum of W*H*D over
@@ -604,7 +604,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=56)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--workload", default="rig1080", choices=["rig1080", "mixed32", "large", "serve"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline only (no mixed32 / serve blocks)")
