@@ -75,6 +75,7 @@ int msum_blocks(const RigDesc& g, int side);   // side 0 row runs, 1 column runs
 // fused vertical sweep: owned columns per block for a group whose rigs have
 // `cols` padded columns in all and disparity count D (0: not used)
 int vsweep_pick_cols(long cols, int D, int choice);
+bool vsweep_in_place(int D);
 int vsweep_blocks(const RigDesc& g, int vcols);
 // per-device set-up of kernel attributes (dynamic LDS above 64 KB); call with
 // the device current.  0 on success.

@@ -36,13 +36,12 @@ struct RigDesc {
     uint8_t* part[4];         // per chain family (rows, cols, diag, anti): for rows
                               // 1..H'-2 the two directions' (L - M) summed, in [0,200];
                               // the rows family keeps its forward (L - M) here and its
-                              // backward one in rowsb; with the fused vertical sweep
-                              // part[1..2] hold the sweep's u16 partial sums
-    uint8_t* rowsb;           // rows family's backward (L - M), rows 1..H'-2: part[3] ahead
-                              // of the fused vertical sweep, a volume of its own otherwise
+                              // backward one in rowsb.  Ahead of the fused vertical sweep
+                              // only part[0] and rowsb exist (part[1..3] null), and the
+                              // sweep's u16 partial sums P replace their bytes row by row
+    uint8_t* rowsb;           // rows family's backward (L - M), rows 1..H'-2
     uint32_t* vstate;         // fused vertical sweep: segment-end states, normalised
                               // (L - M) bytes and minL': [2][3][W'][Dp + 16] bytes
-                              // (the median scratch, free by then)
     uint16_t* edge;           // exact family contributions on rows 0 and H'-1:
                               // [family][2][W'][Dp]
     uint16_t* sum;            // SGM aggregate S, H' x W' x Dp (debug only, may be null)

@@ -36,16 +36,22 @@ namespace {
 
 // workspace layout of one rig (offsets from the workspace base)
 struct WsLayout {
-    size_t code, wta, med, mask, msum, part, edge, scratch, sum, rowsb, total;
+    size_t code, wta, med, mask, msum, part, nvol, vol, edge, scratch, vstate, sum, total;
 };
 
 inline int padded_depth(int D) { return (D + 15) & ~15; }
 
-// keep_sum adds the debug copy of S; rows_b the rows family's backward bytes
-// of the separate chain families (the fused vertical sweep keeps them in part
-// volume 3).  Footprints for admission and sister_workspace_bytes take the
-// default, the larger of the two.
-WsLayout ws_layout(int W, int H, int D, bool keep_sum = false, bool rows_b = true)
+// keep_sum adds the debug copy of S.  separate: the per-chain SGM families,
+// five volumes of Dp bytes per pixel (rows forward, columns, diagonals,
+// anti-diagonals forward, rows backward); otherwise the fused vertical sweep,
+// two (rows forward and backward, which its partial sums then replace) or,
+// for disparity counts with padding lanes and for the debug sweep, four (the
+// partial sums in two of their own, between the rows family's forward and
+// backward volumes).  The
+// serial median's order statistics (prepass form) alias the volumes, which
+// nothing writes before the SGM.  Footprints for admission and
+// sister_workspace_bytes take the default, the larger form.
+WsLayout ws_layout(int W, int H, int D, bool keep_sum = false, bool separate = true)
 {
     const size_t Wp = (size_t)W + 2 * D, Hp = (size_t)H + 2 * D;
     const size_t npx = Wp * Hp, ncell = npx * padded_depth(D);
@@ -56,11 +62,14 @@ WsLayout ws_layout(int W, int H, int D, bool keep_sum = false, bool rows_b = tru
     l.med = o;     o += align_up(8 * npx * sizeof(uint16_t));
     l.mask = o;    o += align_up(4 * npx);
     l.msum = o;    o += align_up(ncell);
-    l.part = o;    o += 4 * align_up(ncell);
+    // the debug sweep (keep_sum) keeps its partial sums in volumes of their own
+    l.nvol = separate ? 5 : (vsweep_in_place(D) && !keep_sum) ? 2 : 4;
+    l.vol = align_up(ncell);
+    l.part = o;    o += std::max(l.nvol * l.vol, align_up(8 * npx * sizeof(uint64_t)));
+    l.scratch = l.part;
     l.edge = o;    o += align_up(4 * 2 * Wp * padded_depth(D) * sizeof(uint16_t));
-    l.scratch = o; o += align_up(8 * npx * sizeof(uint64_t));
+    l.vstate = o;  o += align_up(2 * 3 * Wp * (padded_depth(D) + 16));
     l.sum = o;     if (keep_sum) o += align_up(ncell * sizeof(uint16_t));
-    l.rowsb = o;   if (rows_b) o += align_up(ncell);
     l.total = o;
     return l;
 }
@@ -448,8 +457,7 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Car
         groups.push_back(G);
         k = e;
     }
-    // workspaces: a rig of a fused group keeps the rows family's backward
-    // bytes in part volume 3, the others in a volume of their own
+    // workspaces: a rig of a fused group has two SGM volumes, the others five
     std::vector<WsLayout> lay(n);
     for (const GroupHost& G : groups)
         for (int k = G.i0; k < G.i1; k++) {
@@ -466,13 +474,12 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Car
                 g.med[q] = (uint16_t*)(b.ptr + l.med) + q * npx;
             }
             for (int q = 0; q < 4; q++) g.mask[q] = (uint8_t*)(b.ptr + l.mask) + q * npx;
-            const size_t ncell = npx * g.Dp;
             g.msum = (uint8_t*)(b.ptr + l.msum);
-            for (int q = 0; q < 4; q++) g.part[q] = (uint8_t*)(b.ptr + l.part) + q * align_up(ncell);
-            g.rowsb = G.vcols > 0 ? g.part[3] : (uint8_t*)(b.ptr + l.rowsb);
+            uint8_t* const vols = (uint8_t*)(b.ptr + l.part);
+            for (int q = 0; q < 4; q++) g.part[q] = q < (int)l.nvol - 1 ? vols + q * l.vol : nullptr;
+            g.rowsb = vols + (l.nvol - 1) * l.vol;
             g.edge = (uint16_t*)(b.ptr + l.edge);
-            // 6 W' (Dp + 16) bytes <= 64 W' H' (H' >= 2 D + 4, Dp <= D + 8)
-            g.vstate = (uint32_t*)(b.ptr + l.scratch);
+            g.vstate = (uint32_t*)(b.ptr + l.vstate);
             g.sum = c->keep_sum ? (uint16_t*)(b.ptr + l.sum) : nullptr;
         }
     if ((int)groups.size() > kMaxDGroups) {
