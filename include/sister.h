@@ -186,6 +186,11 @@ int sister_debug_keep_sum(sister_ctx* ctx, int on);
    -1 = automatic (fused for batches wide enough to fill the device).
    Results are identical either way. */
 int sister_debug_vsweep(sister_ctx* ctx, int mode);
+/* Plans built afterwards keep the fused vertical sweep's partial sums over
+   the rows family's bytes, the smaller layout (2.29 against 3.59 GB per
+   1920x1080 D=192 rig, 0.7% slower) that plans take by themselves when they
+   do not fit otherwise.  Results are identical either way. */
+int sister_debug_sweep_in_place(sister_ctx* ctx, int on);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,7 @@ _SIGS = {
     "sister_debug_fetch": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, C.c_void_p, C.c_size_t]),
     "sister_debug_keep_sum": (C.c_int, [C.c_void_p, C.c_int]),
     "sister_debug_vsweep": (C.c_int, [C.c_void_p, C.c_int]),
+    "sister_debug_sweep_in_place": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 
@@ -282,6 +283,11 @@ class Context:
     def vsweep(self, mode):
         """Vertical SGM families: 0 separate sweeps, 1 fused sweep, -1 automatic."""
         self._check(lib().sister_debug_vsweep(self._h, int(mode)))
+
+    def sweep_in_place(self, on=True):
+        """Plans keep the fused sweep's partial sums over the rows family's
+        bytes (the layout of plans short of memory)."""
+        self._check(lib().sister_debug_sweep_in_place(self._h, 1 if on else 0))
 
     def fetch(self, index, what, which, shape, dtype):
         a = np.empty(shape, dtype)

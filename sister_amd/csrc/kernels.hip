@@ -1699,9 +1699,9 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
 // (owned columns stored at the end, owned + ghost columns loaded at the start).
 // No block ever waits for another.
 // Buffers: the rows family's two directions in part[0] and rowsb (bytes,
-// added as words: each sum is at most 200), P (u16) over those same bytes
-// once the down pass has read them when D = 16 KL CH (in part[1..2]
-// otherwise and in the KEEP variant; pd_at below), the segment states in vstate
+// added as words: each sum is at most 200), P (u16) in part[1..2], or over
+// the rows bytes once the down pass has read them in plans short of memory
+// (pd_at below), the segment states in vstate
 // ([parity][cols, diag, anti][W'][Dp + 16] bytes: the normalised (L - M)
 // bytes, then minL').
 // ---------------------------------------------------------------------------
@@ -1836,32 +1836,41 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     // (in bounds, no exec-masked loads or zero fills in the row loop); such a
     // column's states are reset to guards and nothing of it is stored
     const int xc = min(max(x, 0), Wp - 1);
-    // P is this kernel's own intermediate (rows 1..H'-2).  FULL: it takes
-    // the place of the rows family's bytes of the same row and columns, which
-    // the down pass reads just before it stores P: the 2 Dp bytes of column x
-    // occupy its Dp part[0] bytes and its Dp rowsb bytes, and only the wave
-    // that owns column x reads or writes them; its loads of the row's bytes
-    // complete before its P stores (P depends on them).  A lane's 16-byte
-    // piece k = 2 q + h (chunk slot q, half h) lies in part[0] for k < CH, in
-    // rowsb above.  Waves whose columns all lie inside the row use whole-wave
-    // planes: plane k of the wave's 64 / KL columns is 1 KB with wave lane l
-    // at byte 16 l, planes 0..CH-1 over the wave's part[0] bytes of the row
-    // and CH..2CH-1 over its rowsb bytes, so one access moves 1 KB contiguous
-    // (3 ms faster than the cell order at 16 rigs).  The others (the last
-    // wave of a row) keep a column's pieces together: piece k of lane lid at
-    // 16 (lid CH + k mod CH) in the column's bytes of its half.  Either way
-    // the address is a uniform base plus a loop-invariant lane offset.
-    // Otherwise (chunk padding, or the KEEP variant) P has volumes of its
-    // own, part[1..2], in cell order (chunk qg of cell (y, x) at
-    // (y W' + x) Dp + 16 qg).
+    // P is this kernel's own intermediate (rows 1..H'-2).  When the plan gives
+    // it volumes of its own (part[1..2]): FULL, planes per (row, chunk slot
+    // q, half h) in which lane (x, lid) holds 8 u16 at byte (x KL + lid) * 16,
+    // so a wave's 64 lanes move 1 KB contiguous per access (3 ms faster than
+    // the cell layout at 16 rigs); otherwise cell layout (chunk qg of cell
+    // (y, x) at (y W' + x) Dp + 16 qg).
+    // Plans short of memory give it none (part[1] null, FULL only): P then
+    // takes the place of the rows family's bytes of the same row and columns,
+    // which the down pass reads just before it stores P.  The 2 Dp bytes of
+    // column x occupy its Dp part[0] bytes and its Dp rowsb bytes, and only
+    // the wave that owns column x reads or writes them; its loads of the row's
+    // bytes complete before its P stores (P depends on them).  A lane's
+    // 16-byte piece k = 2 q + h lies in part[0] for k < CH, in rowsb above.
+    // Waves whose columns all lie inside the row keep whole-wave planes:
+    // plane k of the wave's 64 / KL columns is 1 KB with wave lane l at byte
+    // 16 l, planes 0..CH-1 over the wave's part[0] bytes of the row and
+    // CH..2CH-1 over its rowsb bytes.  The others (the last wave of a row)
+    // keep a column's pieces together: piece k of lane lid at 16 (lid CH +
+    // k mod CH) in the column's bytes of its half.  (1.3 ms slower per
+    // 64-rig step than volumes of its own.)  Either way the address is a
+    // uniform base plus a loop-invariant lane offset.
+    uint16_t* const Pd = (uint16_t*)g.part[1];
+    const bool p_own = Pd != nullptr;
     constexpr int CPW = 64 / KL;                       // columns per wave
     const int lw = tid & 63;
     const int x0w = __builtin_amdgcn_readfirstlane(x - lw / KL);   // the wave's first column
     const bool wave_planes = FULL && x0w >= 0 && x0w + CPW <= Wp;
-    const uint32_t pd_lane = wave_planes ? (uint32_t)lw * 16u : (uint32_t)(xc * Dp + cg.lid * CH * 16);
-    uint16_t* const Pd = (uint16_t*)g.part[1];
+    const uint32_t pd_lane = p_own ? (uint32_t)(xc * KL + cg.lid) * 16u
+                           : wave_planes ? (uint32_t)lw * 16u : (uint32_t)(xc * Dp + cg.lid * CH * 16);
     auto pd_at = [&](int y, int q, int h) __attribute__((always_inline)) {
         if (FULL) {
+            if (p_own) {
+                const uint16_t* plane = Pd + (((size_t)y * CH + q) * 2 + h) * ((size_t)Wp * KL * 8);   // uniform
+                return (uint16_t*)((const char*)plane + pd_lane);
+            }
             const int k = 2 * q + h, r = k >= CH ? 1 : 0, kk = k - r * CH;
             const size_t s = (size_t)y * Wp * Dp + (wave_planes ? (size_t)x0w * Dp + kk * 1024 : (size_t)kk * 16);
             return (uint16_t*)((r ? g.rowsb : g.part[0]) + s + pd_lane);
@@ -2471,7 +2480,7 @@ static int vs_kl(int D)
 }
 static int vs_ch(int D) { const int kl = vs_kl(D); return kl ? ((D + 15) / 16 + kl - 1) / kl : 0; }
 
-// the fused sweep keeps P over the rows family's bytes (no volumes of its own)
+// the fused sweep can keep P over the rows family's bytes (no volumes of its own)
 bool vsweep_in_place(int D)
 {
     const int kl = vs_kl(D);

@@ -44,14 +44,13 @@ inline int padded_depth(int D) { return (D + 15) & ~15; }
 // keep_sum adds the debug copy of S.  separate: the per-chain SGM families,
 // five volumes of Dp bytes per pixel (rows forward, columns, diagonals,
 // anti-diagonals forward, rows backward); otherwise the fused vertical sweep,
-// two (rows forward and backward, which its partial sums then replace) or,
-// for disparity counts with padding lanes and for the debug sweep, four (the
-// partial sums in two of their own, between the rows family's forward and
-// backward volumes).  The
-// serial median's order statistics (prepass form) alias the volumes, which
-// nothing writes before the SGM.  Footprints for admission and
-// sister_workspace_bytes take the default, the larger form.
-WsLayout ws_layout(int W, int H, int D, bool keep_sum = false, bool separate = true)
+// four (the rows family's forward bytes, its partial sums P in two, the rows
+// family's backward bytes) or, in_place (plans short of memory; only for
+// D = 16 KL CH and not in the debug sweep), two: P then replaces the rows
+// bytes row by row.  The serial median's order statistics (prepass form)
+// alias the volumes, which nothing writes before the SGM.  Footprints for
+// admission and sister_workspace_bytes take the default, the largest form.
+WsLayout ws_layout(int W, int H, int D, bool keep_sum = false, bool separate = true, bool in_place = false)
 {
     const size_t Wp = (size_t)W + 2 * D, Hp = (size_t)H + 2 * D;
     const size_t npx = Wp * Hp, ncell = npx * padded_depth(D);
@@ -62,8 +61,7 @@ WsLayout ws_layout(int W, int H, int D, bool keep_sum = false, bool separate = t
     l.med = o;     o += align_up(8 * npx * sizeof(uint16_t));
     l.mask = o;    o += align_up(4 * npx);
     l.msum = o;    o += align_up(ncell);
-    // the debug sweep (keep_sum) keeps its partial sums in volumes of their own
-    l.nvol = separate ? 5 : (vsweep_in_place(D) && !keep_sum) ? 2 : 4;
+    l.nvol = separate ? 5 : (in_place && vsweep_in_place(D) && !keep_sum) ? 2 : 4;
     l.vol = align_up(ncell);
     l.part = o;    o += std::max(l.nvol * l.vol, align_up(8 * npx * sizeof(uint64_t)));
     l.scratch = l.part;
@@ -273,6 +271,7 @@ struct sister_ctx {
     bool profiling = false;
     bool keep_sum = false;
     int vsweep = -1;              // vertical SGM families: -1 automatic, 0 separate, 1 fused
+    bool p_in_place = false;      // debug: every plan keeps the fused sweep's P in place
     Plan last;                 // plan of the last sister_run_device (kept for debug fetch)
     Plan noalloc;              // plan of the last sister_run_device_noalloc (in the scratch array)
     std::map<GroupKey, GroupForm> gforms;   // autotuned group forms (under mu)
@@ -382,7 +381,11 @@ static void record_latency(sister_ctx* c, double ms)
 // Build the device plan for n rigs whose views/outputs are device pointers.
 // The rig table is ordered by D so that the SGM launches can take each group
 // of equal depth as one contiguous slice.
-static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Carve* carve = nullptr)
+// p_in_place: fused groups keep their partial sums over the rows family's
+// bytes (smaller workspaces, 0.7% slower): the retry of a plan that does not
+// fit otherwise
+static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Carve* carve = nullptr,
+                      bool p_in_place = false)
 {
     p.host_rigs.clear();
     p.workspaces.clear();
@@ -457,12 +460,13 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Car
         groups.push_back(G);
         k = e;
     }
-    // workspaces: a rig of a fused group has two SGM volumes, the others five
+    // workspaces: a rig of a fused group has four SGM volumes (two in place),
+    // the others five
     std::vector<WsLayout> lay(n);
     for (const GroupHost& G : groups)
         for (int k = G.i0; k < G.i1; k++) {
             RigDesc& g = p.host_rigs[k];
-            const WsLayout l = ws_layout(g.W, g.H, g.D, c->keep_sum, G.vcols == 0);
+            const WsLayout l = ws_layout(g.W, g.H, g.D, c->keep_sum, G.vcols == 0, p_in_place || c->p_in_place);
             lay[k] = l;
             Block b;
             if (get(l.total, b) != 0) return no_room("workspace of " + std::to_string(l.total) + " bytes");
@@ -572,12 +576,17 @@ static int build_plan(sister_ctx* c, const sister_rig* rigs, int n, Plan& p, Car
 // split into chunks too small for the plan (earlier plans of other shapes),
 // its entirely free chunks go back to the device and the plan is built again
 // once.  (hipFree synchronises the device; this runs only on that failure.)
+// A plan that does not fit: first return the arena's entirely free chunks
+// (its free memory may sit in chunks shaped for earlier plans) and build
+// again, then build with the fused sweep's partial sums in place.
 static int build_plan_compact(sister_ctx* c, const sister_rig* rigs, int n, Plan& p)
 {
     int rc = build_plan(c, rigs, n, p);
     if (rc != SISTER_ENOMEM) return rc;
     c->pool.trim();
-    return build_plan(c, rigs, n, p);
+    rc = build_plan(c, rigs, n, p);
+    if (rc != SISTER_ENOMEM) return rc;
+    return build_plan(c, rigs, n, p, nullptr, true);
 }
 
 static void free_plan(sister_ctx* c, Plan& p)
@@ -773,6 +782,10 @@ int sister_run_device_noalloc(sister_ctx* c, const sister_rig* rigs, int n, int 
     free_plan(c, c->noalloc);
     Carve carve{c->scratch, kScratchBytes, 0};
     rc = build_plan(c, rigs, n, c->noalloc, &carve);
+    if (rc == SISTER_ENOMEM) {
+        carve = Carve{c->scratch, kScratchBytes, 0};
+        rc = build_plan(c, rigs, n, c->noalloc, &carve, true);
+    }
     if (rc) return rc;
     enqueue(c, c->noalloc, modes, c->stream, c->profiling ? nullptr : &c->fam[0], c->profiling ? &c->timer : nullptr);
     HIPCHK(c, hipGetLastError());
@@ -928,9 +941,9 @@ int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes
     std::vector<sister_rig> part[sister_ctx::kMaxParts];
     int stagger = 0;
     const int nparts = partition_batch(c, rigs, n, modes, part, &stagger);
-    auto build_parts = [&]() {
+    auto build_parts = [&](bool in_place) {
         for (int h = 0; h < nparts; h++) {
-            const int r = build_plan(c, part[h].data(), (int)part[h].size(), c->graph_plan[h]);
+            const int r = build_plan(c, part[h].data(), (int)part[h].size(), c->graph_plan[h], nullptr, in_place);
             if (r) {
                 for (auto& gp : c->graph_plan) free_plan(c, gp);
                 return r;
@@ -938,12 +951,14 @@ int sister_graph_capture(sister_ctx* c, const sister_rig* rigs, int n, int modes
         }
         return (int)SISTER_OK;
     };
-    rc = build_parts();
+    rc = build_parts(false);
     if (rc == SISTER_ENOMEM) {
         // the arena's free memory may sit in chunks shaped for earlier plans:
-        // return the entirely free ones to the device and build again once
+        // return the entirely free ones to the device and build again, then
+        // with the fused sweep's partial sums in place
         c->pool.trim();
-        rc = build_parts();
+        rc = build_parts(false);
+        if (rc == SISTER_ENOMEM) rc = build_parts(true);
     }
     if (rc) return rc;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1262,6 +1277,14 @@ int sister_debug_vsweep(sister_ctx* c, int mode)
     return SISTER_OK;
 }
 
+// debug: the fused sweep's partial sums in place in plans built from now on
+int sister_debug_sweep_in_place(sister_ctx* c, int on)
+{
+    if (!c) return SISTER_EINVAL;
+    c->p_in_place = on != 0;
+    return SISTER_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -1424,7 +1447,8 @@ static void serve_loop(sister_ctx* c)
                 dev[i] = d;
             }
             if (ok) {
-                const int rc = build_plan(c, dev.data(), (int)dev.size(), f->plan);
+                int rc = build_plan(c, dev.data(), (int)dev.size(), f->plan);
+                if (rc == SISTER_ENOMEM) rc = build_plan(c, dev.data(), (int)dev.size(), f->plan, nullptr, true);
                 ok = rc == SISTER_OK;
                 no_memory = rc == SISTER_ENOMEM;
                 if (!ok) fail_reason = plan_error;

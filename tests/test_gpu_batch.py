@@ -326,10 +326,14 @@ def test_full_1080_d192_fused_vertical_equals_separate(gpu_ctx):
         sep = gpu_ctx.compute_rig(rig["views"], rig["D"], modes=7)
         gpu_ctx.vsweep(1)
         fused = gpu_ctx.compute_rig(rig["views"], rig["D"], modes=7)
+        gpu_ctx.sweep_in_place(True)   # partial sums over the rows family's bytes
+        in_place = gpu_ctx.compute_rig(rig["views"], rig["D"], modes=7)
     finally:
         gpu_ctx.vsweep(-1)
+        gpu_ctx.sweep_in_place(False)
     for k in ("multiview", "horizontal", "vertical", "disp", "subpix"):
         assert np.array_equal(fused[k], sep[k]), (k, int((fused[k] != sep[k]).sum()))
+        assert np.array_equal(in_place[k], sep[k]), (k, int((in_place[k] != sep[k]).sum()))
 
 
 def test_serving_caller_errors_while_serving():

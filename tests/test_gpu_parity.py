@@ -125,6 +125,30 @@ def test_fused_vertical_full_lanes_outputs(gpu_ctx, oracle_mod, W, H, D, seed):
     assert float(np.abs(got["subpix"] - ref["subpix"]).max()) <= 1e-3
 
 
+# the fused sweep's partial sums kept over the rows family's bytes (the
+# layout plans short of memory take): outputs, for rows whose last wave of
+# columns is partial (W' not a multiple of the 64 / KL columns of a wave) and
+# for several blocks, with 4 and 8 lanes per column
+@pytest.mark.parametrize("W,H,D,seed,modes", [(96, 64, 64, 13, 7), (100, 44, 64, 17, 7), (68, 40, 128, 18, 7),
+                                              (72, 40, 192, 19, 7), (400, 40, 64, 16, 7), (20, 16, 256, 38, 1),
+                                              (12, 16, 384, 39, 1), (200, 16, 256, 37, 1)])
+def test_fused_vertical_in_place_outputs(gpu_ctx, oracle_mod, W, H, D, seed, modes):
+    print(f"[parity fused in place] W={W} H={H} D={D} seed={seed}", flush=True)
+    rig = synth.rig_small(W, H, D, seed)
+    ref = oracle_mod.compute(rig["views"], D, modes=modes)
+    gpu_ctx.vsweep(1)
+    gpu_ctx.sweep_in_place(True)
+    try:
+        got = gpu_ctx.compute_rig(rig["views"], D, modes=modes)
+    finally:
+        gpu_ctx.vsweep(-1)
+        gpu_ctx.sweep_in_place(False)
+    for k in ("multiview", "horizontal", "vertical", "disp"):
+        if k in ref:
+            assert np.array_equal(got[k], ref[k]), (k, int((got[k] != ref[k]).sum()))
+    assert float(np.abs(got["subpix"] - ref["subpix"]).max()) <= 1e-3
+
+
 # the fused sweep's 8-lane form (D in (192, 384]: 64 owned columns, 2 or 3
 # chunks per lane): every stage (debug sums kept) and the outputs of its
 # full-lane and partial forms, one and several blocks
