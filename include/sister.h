@@ -182,7 +182,7 @@ int sister_debug_fetch(sister_ctx* ctx, int index, const char* what, int which, 
 int sister_debug_keep_sum(sister_ctx* ctx, int on);
 /* Form of the SGM vertical families (columns, diagonals, anti-diagonals) in
    plans built afterwards: 0 = one chain sweep per family, 1 = the fused
-   vertical sweep wherever the disparity count allows it (D <= 192),
+   vertical sweep wherever the disparity count allows it (D <= 384),
    -1 = automatic (fused for batches wide enough to fill the device).
    Results are identical either way. */
 int sister_debug_vsweep(sister_ctx* ctx, int mode);

@@ -1164,7 +1164,10 @@ int sister_autotune(sister_ctx* c, const sister_rig* rigs, int n, int modes, con
         GroupForm f;
         cand.push_back(f);                                       // automatic
         f.fused = 0; cand.push_back(f);                          // separate families, automatic lanes
-        if (k.D <= 192) { f.fused = 1; cand.push_back(f); }      // fused vertical sweep
+        if (vsweep_pick_cols(1, k.D, 1) > 0) {                   // fused vertical sweep, where it reaches
+            f.fused = 1;
+            cand.push_back(f);
+        }
         f = GroupForm(); f.fused = 0;
         for (int q = 0; q < 4; q++) f.kl[q] = 16;
         cand.push_back(f);                                       // separate, 16 lanes everywhere
