@@ -1517,6 +1517,9 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                 }
                 if (!act) return;
                 const bool edge_row = top || bot;
+                // its bytes are read only where S is: at the crop, or everywhere
+                // with the debug sums
+                const bool crop = keep_sum || (y >= D && y < D + g.H && x >= D && x < D + g.W);
                 static_for<0, CH>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
                     if (cg.lid * CH + q >= cg.nq) return;
@@ -1531,7 +1534,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                         }
                         ST16(e, a);
                         ST16(e + 8, bb);
-                    } else {
+                    } else if (crop) {
                         ST16((dir == 1 ? g.rowsb : part) + off + 16 * q, pack16(tp[q]));
                     }
                 });
@@ -1862,6 +1865,11 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
     constexpr int CPW = 64 / KL;                       // columns per wave
     const int lw = tid & 63;
     const int x0w = __builtin_amdgcn_readfirstlane(x - lw / KL);   // the wave's first column
+    // S, and so P and the rows family's bytes it is formed from, is read only
+    // at the crop (D <= y < D + H, D <= x < D + W) outside the debug variant:
+    // rows outside it, and waves whose columns all lie outside it, neither
+    // load nor store them (the family steps run everywhere)
+    const bool wave_crop = KEEP || (x0w + CPW > D && x0w < D + g.W);
     const bool wave_planes = FULL && x0w >= 0 && x0w + CPW <= Wp;
     const uint32_t pd_lane = p_own ? (uint32_t)(xc * KL + cg.lid) * 16u
                            : wave_planes ? (uint32_t)lw * 16u : (uint32_t)(xc * Dp + cg.lid * CH * 16);
@@ -2037,11 +2045,12 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
         }
         const bool interior = y >= 1 && y <= Hp - 2;
         const bool edge_row = !interior;
+        const bool pc = wave_crop && (KEEP || (y >= D && y < D + g.H));   // P and rows bytes needed
         const size_t cell = ((size_t)y * Wp + x) * Dp + boff;
         // addends of the row's output, loaded now and used after the three steps
         uint4 ob[2 * CH];
         static_for<0, 2 * CH>([&](auto kc) { ob[decltype(kc)::value] = make_uint4(0, 0, 0, 0); });
-        if (own && act && interior) {
+        if (own && act && interior && pc) {
             static_for<0, CH>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 if (!FULL && cg.lid * CH + q >= cg.nq) return;
@@ -2094,6 +2103,7 @@ __global__ __launch_bounds__(kVMaxThreads) void k_vsweep(const RigDesc* __restri
             if (DIR == 1 && keep_sum && g.sum) edge_sum(y);   // y == 0
             continue;
         }
+        if (!pc) continue;
         // acc = (L_cols + L_FL + L_FR) - 3 M (- the pre-step minima) from the new states of this column,
         // one chunk at a time (scheduling barriers keep the compiler from
         // hoisting every chunk's LDS reads at once)
