@@ -6,7 +6,7 @@ mode 0 (multiview).  Two forms are measured on the same synthetic rigs, with
 inputs and outputs resident in HBM before the timed region:
 
   plain    one rig per call of the library (sister_run_device with n = 1),
-  batched  B rigs in one plan (default 64: 243 GB of workspace), captured
+  batched  B rigs in one plan (default 64: 229 GB of workspace), captured
            once as a hipGraph and replayed.
 
 `value` is the batched throughput in G disparity-evals/s (sum of W*H*D over
@@ -65,19 +65,31 @@ OPS_PER_CELL_PIPELINE = 97        # mode 0: costs 20, LR-check argmins 8, weight
 OPS_PER_CELL = {"wta": 28, "msum": 4, "sgm_rows": 16, "sgm_cols": 16, "sgm_diag": 16, "sgm_anti": 17,
                 "sgm_vsweep": 49}   # the fused vertical sweep: 3 families x 2 sweeps x 8, + argmin
 
-# Algorithmic HBM bytes per padded cell of the HBM-bound kernels.  Each SGM
-# chain family reads the u8 cost M in both sweeps (2), writes its forward
-# (L - M) byte (1), reads it back and writes the two-direction sum (2): 5.
-# The anti-diagonal family, which forms S and the winner, instead reads the
-# three other families' bytes (3) and writes nothing in its backward sweep: 7.
-# Ahead of the fused vertical sweep the rows family stores its two directions
-# separately (read M twice, write two bytes): 4.  The fused vertical sweep
-# (columns, diagonals, anti-diagonals together) reads M (1) and the rows
-# family's two bytes (2) and writes the u16 partial sum P (2) going down,
-# then reads M (1) and P (2) going up: 8 (against 5 + 5 + 7 for the three
-# separate families).  The cost sum over 16x16 tiles writes M once (1); its
-# two-launch form (frames 0/180, then 90/270 read-modify-write) moves 3.
-BYTES_PER_CELL = {"sgm_rows": 4, "sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "sgm_vsweep": 8, "msum": 1}
+# Algorithmic HBM bytes per padded cell of the HBM-bound kernels, with
+# `crop` the fraction of padded cells inside the crop (W*H of W'*H'): the
+# intermediates whose only reader is the final aggregate S are moved for those
+# cells alone, since S is read at the crop only.  Each separate SGM chain
+# family reads the u8 cost M in both sweeps (2), writes its forward (L - M)
+# byte (1), reads it back and writes the two-direction sum (2): 5.  The
+# anti-diagonal family, which forms S and the winner, instead reads the three
+# other families' bytes (3) and writes nothing in its backward sweep: 7.  The
+# rows family sweeps its two directions as separate waves, each reading M (2),
+# and stores one byte per direction at crop cells (2 crop).  The fused
+# vertical sweep (columns, diagonals, anti-diagonals together) reads M going
+# down and again going up (2) and, at crop cells, the rows family's two bytes
+# (2), the u16 partial sum P written going down (2) and read going up (2):
+# 2 + 6 crop (against 5 + 5 + 7 for the three separate families).  The cost
+# sum over 16x16 tiles writes M once (1); its two-launch form (frames 0/180,
+# then 90/270 read-modify-write) moves 3.
+def bytes_per_cell(kernel, crop):
+    fixed = {"sgm_cols": 5, "sgm_diag": 5, "sgm_anti": 7, "msum": 1}
+    if kernel == "sgm_rows":
+        return 2 + 2 * crop
+    if kernel == "sgm_vsweep":
+        return 2 + 6 * crop
+    return fixed.get(kernel)
+
+
 # per-kernel counter totals of one instrumented step (used when their batch
 # size is the one measured)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_rig1080_d192.json")
@@ -247,7 +259,7 @@ def run_single(args):
 # ---------------------------------------------------------------------------
 # headline: plain vs batched at 1920x1080 D=192
 # ---------------------------------------------------------------------------
-def roofline_entry(kern, launches, cells, step_ms, B, workload):
+def roofline_entry(kern, launches, cells, step_ms, B, workload, crop):
     traffic = None
     per_kernel_traffic = {}
     valu = {}
@@ -277,9 +289,11 @@ def roofline_entry(kern, launches, cells, step_ms, B, workload):
          "launches_per_step": n, "avg_launch_ms": avg_ms,
          "achieved": ops / n / (avg_ms * 1e-3) / 1e12, "peak": VALU_PEAK / 1e12, "unit": "T int-lane-ops/s",
          "frac": ops / n / (avg_ms * 1e-3) / VALU_PEAK}
-    if dom in BYTES_PER_CELL:
-        alg = BYTES_PER_CELL[dom] * cells
-        d.update({"bytes_per_cell": BYTES_PER_CELL[dom], "hbm_achieved": alg / (kern[dom] * 1e-3) / 1e9,
+    bpc = bytes_per_cell(dom, crop)
+    if bpc is not None:
+        alg = bpc * cells
+        d.update({"bytes_per_cell": bpc, "crop_fraction": crop,
+                  "hbm_achieved": alg / (kern[dom] * 1e-3) / 1e9,
                   "hbm_frac": alg / (kern[dom] * 1e-3) / HBM_PEAK, "hbm_unit": "GB/s (algorithmic bytes)"})
         if dom in per_kernel_traffic:
             d["traffic"] = per_kernel_traffic[dom]
@@ -415,7 +429,7 @@ def run(args):
         "batched_compared": f"disp and multiview maps of all {B} rigs against their scene computed alone",
         "padded_cells_per_step": cells,
         "padded_cells_per_s": cells / (step_ms * 1e-3),
-        "roofline": roofline_entry(kern, launches, cells, step_ms, B, args.workload),
+        "roofline": roofline_entry(kern, launches, cells, step_ms, B, args.workload, evals / cells),
         "kernel_ms": kern,
         "kernel_launches_per_step": launches,
     }

@@ -713,10 +713,11 @@ void sister_destroy(sister_ctx* c)
     c->pool.release_all();
     c->pinned.release_all();
     (void)hipEventDestroy(c->ev_fork);
-    for (int k = 1; k < sister_ctx::kMaxParts; k++) {
-        (void)hipEventDestroy(c->ev_join[k]);
-        (void)hipEventDestroy(c->ev_stage[k]);
-        (void)hipStreamDestroy(c->side[k]);
+    for (int k = 0; k < sister_ctx::kMaxParts; k++) {
+        if (c->ev_stage[k]) (void)hipEventDestroy(c->ev_stage[k]);
+        if (k == 0) continue;   // part 0 runs on the context stream
+        if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
     }
     for (auto& f : c->fam) {
         (void)hipEventDestroy(f.fork);
