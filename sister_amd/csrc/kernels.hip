@@ -1290,12 +1290,18 @@ __device__ __forceinline__ void lane_step_f(LaneState<CH>& st, const ChainGeo& c
     const uint32_t rfill = (cg.lid == KL - 1) ? kGuard2 : nbr;
     const uint32_t mpp = dup16(st.mp);
     const us2 p2t = U2(mpp) + (us2){(unsigned short)kP2, (unsigned short)kP2};
+    // truncation and padding as one and-or per pair: every new value is below
+    // 0x4000 (at most P2 + 255 normalised, raw values stay below the guard), so
+    // or-ing the guard 0x3FFF into a padding pair yields the guard itself
+    const uint32_t tmask = trunc ? 0x00FF00FFu : 0xFFFFFFFFu;
     us2 rmin = U2(0xFFFFFFFFu);
     static_for<0, CH>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const int qg = cg.lid * CH + q;
         const bool valid = qg < cg.nq;
         const bool ht = cg.half_tail && qg == cg.nq - 1;
+        const uint32_t pad_lo = valid ? 0u : kGuard2;           // pairs 0..3 of the chunk
+        const uint32_t pad_hi = valid && !ht ? 0u : kGuard2;    // pairs 4..7
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const int j = 8 * q + i;
@@ -1306,10 +1312,7 @@ __device__ __forceinline__ void lane_step_f(LaneState<CH>& st, const ChainGeo& c
             const us2 nb = umin2(left, right) + (us2){(unsigned short)kP1, (unsigned short)kP1};
             const us2 t = umin2(umin2(nb, U2(cur)), p2t);
             const us2 d = DEFER ? t : t - U2(mpp);
-            us2 l = d + U2(mpair(q, i));
-            if (trunc) l = l & (us2){0xFF, 0xFF};
-            const bool pad = !valid || (ht && i >= 4);
-            l = pad ? U2(kGuard2) : l;
+            const us2 l = U2((W2(d + U2(mpair(q, i))) & tmask) | (i < 4 ? pad_lo : pad_hi));
             tp[q][i] = ACC ? W2(U2(tp[q][i]) + d) : W2(d);
             st.L[j] = W2(l);
             prv = cur;
@@ -1417,6 +1420,8 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
     uint16_t* edge = g.edge + (size_t)FAM * 2 * (size_t)Wp * Dp;
     const int boff = cg.lid * CH * 16;     // byte offset of the lane's chunks in a cell
     const bool final_pass = FAM == 3;
+    // rows family: byte offset of the lane's chunks in column 0 of its row
+    const size_t row_off = FAM == 0 ? (size_t)(valid ? c : 0) * Wp * Dp + boff : 0;
 
     // ---- chain geometry --------------------------------------------------
     // steps 0..n-1 of the wave; cell of chain c at step t, per direction
@@ -1466,6 +1471,17 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
         const bool need_p = dir == 1 && FAM != 0;
         const bool need_o = final_pass && dir == 1;
         auto fetch = [&](int t, RowBuf<CH, NO>& b) __attribute__((always_inline)) {
+            if constexpr (FAM == 0) {
+                // rows: the lane's row offset is fixed, the column uniform
+                const int x = dir == 0 ? t : Wp - 1 - t;
+                const uint8_t* src = g.msum + row_off + (size_t)x * Dp;
+                static_for<0, CH>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    const bool ok = t < nsteps && valid && cg.lid * CH + q < cg.nq;
+                    b.m[q] = ok ? LD16(src + 16 * q) : make_uint4(0, 0, 0, 0);
+                });
+                return;
+            }
             int y, x;
             cell_of(dir, t, y, x);
             const bool act = t < nsteps && is_active(y, x);
@@ -1535,7 +1551,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                         ST16(e, a);
                         ST16(e + 8, bb);
                     } else if (crop) {
-                        ST16((dir == 1 ? g.rowsb : part) + off + 16 * q, pack16(tp[q]));
+                        ST16((dir == 1 ? g.rowsb : part) + row_off + (size_t)x * Dp + 16 * q, pack16(tp[q]));
                     }
                 });
                 return;
