@@ -9,6 +9,7 @@ enum Stage {
     kStageCensus = 0,
     kStageWta,          // all WTA blocks, or (row form) the rows of frames 0/2
     kStageWtaV,         // row form: the rows of frames 1/3
+    kStageWtaFill,      // the WTA maps' deep padding bands, copied from their first rows
     kStageMedianPrep,
     kStageMask,
     kNumStages

@@ -146,7 +146,7 @@ __device__ __forceinline__ unsigned wave_min(unsigned v)
 // ---------------------------------------------------------------------------
 constexpr int kCT = 32;   // tile side (frame pixels)
 
-__global__ __launch_bounds__(256) void k_census(const RigDesc* __restrict__ rigs, int nrig,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_census(const RigDesc* __restrict__ rigs, int nrig,
                                                 const int* __restrict__ bs)
 {
     constexpr int TR = kCT + 6;       // frame rows r0-3 .. r0+kCT+2
@@ -174,59 +174,96 @@ __global__ __launch_bounds__(256) void k_census(const RigDesc* __restrict__ rigs
         tb[rr * TW * 4 + cc] = (r >= 0 && r < R && c >= 0 && c < C) ? (uint8_t)frame_px(g, v, F, r, c) : 0;
     }
     __syncthreads();
-    // run of pixels (r, cb .. cb+3); C is a multiple of 4, so a run is all in or all out
-    const int run = threadIdx.x & 7, rl = threadIdx.x >> 3;
-    const int r = r0 + rl, cb = c0 + 4 * run;
-    if (r >= R || cb >= C) return;
-    // w[i][k]: dword run + k of tile row rl + i (frame row r - 3 + i); byte b of
-    // the run's window is frame column cb - 8 + b
-    uint32_t w[7][4];
+    // codes of the run of pixels (r0 + rl, c0 + 4 run .. + 3), from the tile
+    auto run_codes = [&](int rl, int run, uint64_t (&code)[4]) __attribute__((always_inline)) {
+        const int r = r0 + rl, cb = c0 + 4 * run;
+        // w[i][k]: dword run + k of tile row rl + i (frame row r - 3 + i); byte b of
+        // the run's window is frame column cb - 8 + b
+        uint32_t w[7][4];
 #pragma unroll
-    for (int i = 0; i < 7; i++)
+        for (int i = 0; i < 7; i++)
 #pragma unroll
-        for (int k = 0; k < 4; k++) w[i][k] = tile[(rl + i) * TW + run + k];
-    // bytes s .. s+3 of row i: byte p is pixel p's entry at window byte s + p
-    auto quad = [&](int i, int sb) __attribute__((always_inline)) {
-        return (sb & 3) == 0 ? w[i][sb >> 2] : __builtin_amdgcn_alignbyte(w[i][(sb >> 2) + 1], w[i][sb >> 2], sb & 3);
-    };
-    // test u (offset (u/9 - 3, u%9 - 4)) and its mirror 62 - u compare the same
-    // two pixels in opposite order: bit 62 - u = (a > b) is the sign of b - a,
-    // bit u = (b > a) the sign of a - b.  hi collects bits 62..32 MSB-first;
-    // lf collects bits 0..30 in the same order and is bit-reversed at the end.
-    uint32_t hi[4] = {0, 0, 0, 0}, lf[4] = {0, 0, 0, 0};
+            for (int k = 0; k < 4; k++) w[i][k] = tile[(rl + i) * TW + run + k];
+        // bytes s .. s+3 of row i: byte p is pixel p's entry at window byte s + p
+        auto quad = [&](int i, int sb) __attribute__((always_inline)) {
+            return (sb & 3) == 0 ? w[i][sb >> 2] : __builtin_amdgcn_alignbyte(w[i][(sb >> 2) + 1], w[i][sb >> 2], sb & 3);
+        };
+        // test u (offset (u/9 - 3, u%9 - 4)) and its mirror 62 - u compare the same
+        // two pixels in opposite order: bit 62 - u = (a > b) is the sign of b - a,
+        // bit u = (b > a) the sign of a - b.  hi collects bits 62..32 MSB-first;
+        // lf collects bits 0..30 in the same order and is bit-reversed at the end.
+        uint32_t hi[4] = {0, 0, 0, 0}, lf[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int u = 0; u < 31; u++) {
-        const int y = u / 9 - 3, x = u % 9 - 4;
-        const uint32_t qa = quad(3 + y, 8 + x), qb = quad(3 - y, 8 - x);
+        for (int u = 0; u < 31; u++) {
+            const int y = u / 9 - 3, x = u % 9 - 4;
+            const uint32_t qa = quad(3 + y, 8 + x), qb = quad(3 - y, 8 - x);
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const uint32_t a = (qa >> (8 * p)) & 0xFFu, b = (qb >> (8 * p)) & 0xFFu;
+                hi[p] = __builtin_amdgcn_alignbit(hi[p], b - a, 31);
+                lf[p] = __builtin_amdgcn_alignbit(lf[p], a - b, 31);
+            }
+        }
+        // carry of pixel c: the (3,4) test of pixel c-1, I(r+3, c+3) > I(r-3, c-5)
+        const uint32_t ca = quad(6, 11 + 0), cbq = quad(0, 3);
 #pragma unroll
         for (int p = 0; p < 4; p++) {
-            const uint32_t a = (qa >> (8 * p)) & 0xFFu, b = (qb >> (8 * p)) & 0xFFu;
-            hi[p] = __builtin_amdgcn_alignbit(hi[p], b - a, 31);
-            lf[p] = __builtin_amdgcn_alignbit(lf[p], a - b, 31);
+            const int c = cb + p;
+            code[p] = 0;
+            if (r >= 3 && r < R - 3 && c >= 4 && c < C - 4) {
+                const uint32_t lo = __builtin_bitreverse32(lf[p]) >> 1;
+                unsigned carry;
+                if (c > 4)
+                    carry = ((ca >> (8 * p)) & 0xFFu) > ((cbq >> (8 * p)) & 0xFFu);
+                else if (r > 3)
+                    carry = frame_px(g, v, F, r + 2, C - 1) > frame_px(g, v, F, r - 4, C - 9);  // (r-1, C-5)
+                else
+                    carry = 0;
+                code[p] = ((uint64_t)(hi[p] | (carry << 31)) << 32) | lo;
+            }
         }
-    }
-    // carry of pixel c: the (3,4) test of pixel c-1, I(r+3, c+3) > I(r-3, c-5)
-    const uint32_t ca = quad(6, 11 + 0), cbq = quad(0, 3);
-    uint64_t code[4];
+    };
+    auto store_run = [&](int r, int cb, const uint64_t (&code)[4]) __attribute__((always_inline)) {
+        uint64_t* out = g.code[unit] + (size_t)r * C + cb;   // 32-byte aligned: cb % 4 == 0
+        ST16(out, make_uint4((uint32_t)code[0], (uint32_t)(code[0] >> 32), (uint32_t)code[1], (uint32_t)(code[1] >> 32)));
+        ST16(out + 2, make_uint4((uint32_t)code[2], (uint32_t)(code[2] >> 32), (uint32_t)code[3], (uint32_t)(code[3] >> 32)));
+    };
+    const int run = threadIdx.x & 7, rl = threadIdx.x >> 3;
+    // Tiles in the deep padding.  With replicate padding the window of a pixel
+    // at frame rows [3, D-4] holds the image's first row only (rows
+    // [R-D+3, R-4]: its last row), so its code does not depend on the row; the
+    // carry of column 4 compares two copies of one pixel there (0, as at row
+    // 3).  Likewise a pixel at columns [5, D-5] or [C-D+4, C-5] sees one image
+    // column and its carry compares two copies of the same column, so within a
+    // row its code does not depend on the column.  A tile wholly inside such a
+    // band computes its distinct codes in one wave (one run of 4 per tile row
+    // band, one pixel per row for a column band, one code for both) and copies.
+    const int D = g.D;
+    const bool rows_deep = (r0 >= 3 && r0 + kCT - 1 <= D - 4) || (r0 >= R - D + 3 && r0 + kCT - 1 <= R - 4);
+    const bool cols_deep = (c0 >= 5 && c0 + kCT - 1 <= D - 5) || (c0 >= C - D + 4 && c0 + kCT - 1 <= C - 5);
+    const bool deep = rows_deep || cols_deep;   // uniform over the block
+    const int nrep = rows_deep ? (cols_deep ? 1 : kCT / 4) : kCT;
+    const bool work = deep ? (int)threadIdx.x < nrep : r0 + rl < R && c0 + 4 * run < C;
+    uint64_t code[4] = {0, 0, 0, 0};
+    if (work)
+        run_codes(deep ? (rows_deep ? 0 : (int)threadIdx.x) : rl,
+                  deep ? (rows_deep && !cols_deep ? (int)threadIdx.x : 0) : run, code);
+    if (deep) {
+        __shared__ uint64_t rep[kCT * 4];
+        if (work) {
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const int c = cb + p;
-        code[p] = 0;
-        if (r >= 3 && r < R - 3 && c >= 4 && c < C - 4) {
-            const uint32_t lo = __builtin_bitreverse32(lf[p]) >> 1;
-            unsigned carry;
-            if (c > 4)
-                carry = ((ca >> (8 * p)) & 0xFFu) > ((cbq >> (8 * p)) & 0xFFu);
-            else if (r > 3)
-                carry = frame_px(g, v, F, r + 2, C - 1) > frame_px(g, v, F, r - 4, C - 9);  // (r-1, C-5)
-            else
-                carry = 0;
-            code[p] = ((uint64_t)(hi[p] | (carry << 31)) << 32) | lo;
+            for (int p = 0; p < 4; p++) rep[threadIdx.x * 4 + p] = code[p];
         }
+        __syncthreads();
+        // a row band's tile may end past column C, a column band's past row R
+        if (r0 + rl >= R || c0 + 4 * run >= C) return;
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+            code[p] = rows_deep ? (cols_deep ? rep[0] : rep[run * 4 + p]) : rep[rl * 4];
+    } else if (!work) {
+        return;   // C is a multiple of 4, so a run is all in or all out
     }
-    uint64_t* out = g.code[unit] + (size_t)r * C + cb;   // 32-byte aligned: cb % 4 == 0
-    ST16(out, make_uint4((uint32_t)code[0], (uint32_t)(code[0] >> 32), (uint32_t)code[1], (uint32_t)(code[1] >> 32)));
-    ST16(out + 2, make_uint4((uint32_t)code[2], (uint32_t)(code[2] >> 32), (uint32_t)code[3], (uint32_t)(code[3] >> 32)));
+    store_run(r0 + rl, c0 + 4 * run, code);
 }
 
 // ---------------------------------------------------------------------------
@@ -238,6 +275,16 @@ __global__ __launch_bounds__(256) void k_census(const RigDesc* __restrict__ rigs
 // Lowest index wins ties: keys are (cost << 9) | d, reduced with min.
 // One block = one frame row segment of 256 pixels; codes staged in LDS.
 // ---------------------------------------------------------------------------
+// Frame rows of the deep padding bands, [3, D-4] and [R-D+3, R-4]: with
+// replicate padding every census code of such a row equals that of the band's
+// first row, in the centre and the neighbour frame alike (the window sees one
+// image row; see k_census), so their WTA maps equal the first row's.  The WTA
+// kernels compute the first row of each band and k_wta_fill copies it.
+__device__ __forceinline__ bool wta_band_copy(int r, int R, int D)
+{
+    return (r > 3 && r <= D - 4) || (r > R - D + 3 && r <= R - 4);
+}
+
 __global__ __launch_bounds__(256) void k_wta(const RigDesc* __restrict__ rigs, int nrig,
                                              const int* __restrict__ bs)
 {
@@ -253,9 +300,9 @@ __global__ __launch_bounds__(256) void k_wta(const RigDesc* __restrict__ rigs, i
     else if ((lb -= n13) < n02) F = 2;
     else { lb -= n02; F = 3; }
     const int R = frame_rows(g, F), C = frame_cols(g, F);
-    (void)R;
     const int tilesC = (C + 255) / 256;
     const int r = lb / tilesC, c0 = (lb % tilesC) * 256;
+    if (wta_band_copy(r, R, D)) return;   // copied from the band's first row
     const uint64_t* cc = g.code[2 * F] + (size_t)r * C;
     const uint64_t* cn = g.code[2 * F + 1] + (size_t)r * C;
     const int n = 256 + D - 1;
@@ -390,6 +437,7 @@ __global__ __launch_bounds__(64 * kWtaRowMaxWaves) void k_wta_row(const RigDesc*
     const int nrow = V ? g.Wp : g.Hp, C = V ? g.Hp : g.Wp;
     const int F = (V ? 1 : 0) + (lb >= nrow ? 2 : 0);
     const int r = lb >= nrow ? lb - nrow : lb;
+    if (wta_band_copy(r, nrow, D)) return;   // copied from the band's first row
     const uint64_t* cc = g.code[2 * F] + (size_t)r * C;
     const uint64_t* cn = g.code[2 * F + 1] + (size_t)r * C;
     uint64_t* ccS = lds64;                       // ccS[c] = CC[c]
@@ -421,6 +469,27 @@ __global__ __launch_bounds__(64 * kWtaRowMaxWaves) void k_wta_row(const RigDesc*
     __syncthreads();
     uint16_t* const outR = g.wta[2 * F + 1] + (size_t)r * C;
     for (int p = threadIdx.x; p < C; p += blockDim.x) G(outR)[p] = (uint16_t)(RP[p] & 511u);
+}
+
+// The WTA maps of the deep padding bands' rows from each band's first row:
+// one block per (map, band, 1024 columns), 4 columns per thread.
+constexpr int kWtaFillCols = 1024;
+__global__ __launch_bounds__(256) void k_wta_fill(const RigDesc* __restrict__ rigs, int nrig,
+                                                  const int* __restrict__ bs)
+{
+    const int ri = find_rig(bs, nrig, blockIdx.x);
+    const RigDesc& g = rigs[ri];
+    const int lb = blockIdx.x - bs[ri];
+    const int nch = (max(g.Wp, g.Hp) + kWtaFillCols - 1) / kWtaFillCols;
+    const int m = lb / (2 * nch), band = (lb / nch) & 1;
+    const int F = m >> 1;
+    const int R = frame_rows(g, F), C = frame_cols(g, F), D = g.D;
+    const int c = (lb % nch) * kWtaFillCols + 4 * (int)threadIdx.x;   // C % 4 == 0
+    if (c >= C) return;
+    const int lo = band ? R - D + 3 : 3, hi = band ? R - 4 : D - 4;
+    uint16_t* const map = g.wta[m];
+    const uint2 v = *(const uint2*)(map + (size_t)lo * C + c);
+    for (int r = lo + 1; r <= hi; r++) *(uint2*)(map + (size_t)r * C + c) = v;
 }
 
 // compile-time loop: f(integral_constant<int, I>) for I in [B, E), so register
@@ -2342,6 +2411,10 @@ void launch_wta(const LaunchSet& ls, hipStream_t st, KernelTimer* t)
         hipLaunchKernelGGL(k_wta, dim3(ls.blocks[kStageWta]), dim3(256), lds, st, ls.rigs, ls.nrig, ls.bs[kStageWta]);
         TD(kKWta, 1);
     }
+    // the deep padding bands' rows, copied from each band's first row
+    hipLaunchKernelGGL(k_wta_fill, dim3(ls.blocks[kStageWtaFill]), dim3(256), 0, st, ls.rigs, ls.nrig,
+                       ls.bs[kStageWtaFill]);
+    TD(kKWta, 1);
     TE(kKWta);
 }
 
@@ -2704,6 +2777,7 @@ void stage_blocks(const RigDesc& g, int maxD, bool wta_rows, int out[kNumStages]
         out[kStageWta] = 2 * g.Hp * ((g.Wp + 255) / 256) + 2 * g.Wp * ((g.Hp + 255) / 256);
         out[kStageWtaV] = 0;
     }
+    out[kStageWtaFill] = 8 * 2 * ((std::max(g.Wp, g.Hp) + kWtaFillCols - 1) / kWtaFillCols);
     out[kStageMedianPrep] = (int)(8 * ((npx + 255) / 256));
     out[kStageMask] = 4 * ((g.Wp + kMT32 - 1) / kMT32) * ((g.Hp + kMT32 - 1) / kMT32);
 }
