@@ -102,6 +102,36 @@ def test_census_borders_zero(oracle_mod):
     assert not c[:3].any() and not c[-3:].any() and not c[:, :4].any() and not c[:, -4:].any()
 
 
+@pytest.mark.parametrize("W,H,D,seed", [(80, 60, 72, 7), (40, 36, 8, 9), (48, 40, 104, 11)])
+def test_deep_padding_bands(oracle_mod, W, H, D, seed):
+    """The identities the GPU census and WTA rely on to copy instead of
+    compute, in every frame of R x C padded pixels: census codes are equal
+    along frame rows [3, D-4] and [R-D+3, R-4] within each column, and along
+    columns [5, D-5] and [C-D+4, C-5] within each row; the WTA maps are equal
+    over those rows.  One row or column further out (the frame border, the
+    column-4 carry) they are not."""
+    rig = synth.rig_small(W, H, D, seed)
+    r = oracle_mod.compute(rig["views"], D, modes=1, debug=True)
+    outer = [False] * 3
+    for u in range(8):
+        c = r["census"][u]
+        R, C = c.shape
+        for lo, hi in ((3, D - 4), (R - D + 3, R - 4)):
+            assert (c[lo:hi + 1] == c[lo]).all(), (u, "rows", lo)
+        for lo, hi in ((5, D - 5), (C - D + 4, C - 5)):
+            assert (c[:, lo:hi + 1] == c[:, lo:lo + 1]).all(), (u, "cols", lo)
+        outer[0] |= not (c[2:D - 3] == c[3]).all()
+        outer[1] |= not (c[:, 4:D - 4] == c[:, 5:6]).all()
+    for F in range(4):
+        for nm in ("wta_l", "wta_r"):
+            m = np.asarray(r[nm][F])
+            R = m.shape[0]
+            for lo, hi in ((3, D - 4), (R - D + 3, R - 4)):
+                assert (m[lo:hi + 1] == m[lo]).all(), (F, nm, lo)
+            outer[2] |= not (m[2:D - 3] == m[3]).all()
+    assert outer[0] and outer[2] and (outer[1] or D - 5 < 5), outer   # D = 8: no column band
+
+
 # ---------------------------------------------------------------------------
 # Hamming cost (census.cpp:54-146) by bit counting
 # ---------------------------------------------------------------------------
