@@ -1442,9 +1442,12 @@ __device__ __forceinline__ uint4 add_u16x8(uint4 a, uint4 b)
 #ifndef SISTER_PF8
 #define SISTER_PF8 2
 #endif
+#ifndef SISTER_PF4
+#define SISTER_PF4 2
+#endif
 template <int CH, int KL, int FAM>
 struct FamTune {
-    static constexpr int PF = KL == 16 ? (FAM == 3 ? SISTER_PF16A : SISTER_PF16) : (KL == 8 ? SISTER_PF8 : 2);
+    static constexpr int PF = KL == 16 ? (FAM == 3 ? SISTER_PF16A : SISTER_PF16) : (KL == 8 ? SISTER_PF8 : SISTER_PF4);
 };
 
 // Family kernel: FAM 0 rows (lanes over rows, steps along x), 1 columns,
