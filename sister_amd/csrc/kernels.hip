@@ -1494,6 +1494,12 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
     const bool final_pass = FAM == 3;
     // rows family: byte offset of the lane's chunks in column 0 of its row
     const size_t row_off = FAM == 0 ? (size_t)(valid ? c : 0) * Wp * Dp + boff : 0;
+    // other families: the chain's cell at row y lies at y (W' + s) Dp + lane_off,
+    // the row term uniform over the wave (s = 0 columns, +1 diagonals, -1
+    // anti-diagonals; lane_off may be negative where the chain starts left of
+    // column 0, and is used only at active cells)
+    const long ystride = (long)(Wp + (FAM == 2 ? 1 : FAM == 3 ? -1 : 0)) * Dp;
+    const long lane_off = (long)(FAM == 2 ? c - (Hp - 1) : c) * Dp + boff;
 
     // ---- chain geometry --------------------------------------------------
     // steps 0..n-1 of the wave; cell of chain c at step t, per direction
@@ -1557,7 +1563,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
             int y, x;
             cell_of(dir, t, y, x);
             const bool act = t < nsteps && is_active(y, x);
-            const size_t off = act ? ((size_t)y * Wp + x) * Dp + boff : 0;
+            const size_t off = (size_t)((long)y * ystride + lane_off);
             static_for<0, CH>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 const bool ok = act && cg.lid * CH + q < cg.nq;
@@ -1577,7 +1583,7 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
             int y, x;
             cell_of(dir, t, y, x);
             const bool act = is_active(y, x);
-            const size_t off = act ? ((size_t)y * Wp + x) * Dp + boff : 0;
+            const size_t off = (size_t)((long)y * ystride + lane_off);   // used at active cells
             uint32_t m[CH][8], tp[CH][8];
             static_for<0, CH>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
