@@ -345,12 +345,15 @@ __global__ __launch_bounds__(256) void k_wta(const RigDesc* __restrict__ rigs, i
 //   key   = (pop(CC[c_i] ^ w[i]) << 9) | d        (lowest index wins ties)
 //   L[i]  = min(L[i], key)                         left map of pixel c_i
 //   R[i]  = min(R[i], key)                         right map of pixel c_i - d
+// kept as their complements ~key under max (one v_mad_i32_i24 forms ~key =
+// ~d - 512 pop), so that 0 is the neutral accumulator.
 // R[i] belongs to pixel x0 + 4l + i - d, so the right accumulators move one
 // pixel per step: R[3] of lane l becomes R[0] of lane l + 1 (DPP wave_shr:1,
-// a fresh accumulator entering lane 0).  Accumulators leaving lane 63, and
-// all of them after the last step, are partial minima (their pixel's other
-// candidates lie in neighbouring segments): they meet by LDS atomic min, once
-// per pixel and segment.  About 3.6 VALU per evaluation against 6.
+// whose zero fill is the fresh accumulator entering lane 0: one move).
+// Accumulators leaving lane 63, and all of them after the last step, are
+// partial results (their pixel's other candidates lie in neighbouring
+// segments): they meet by LDS atomic max, once per pixel and segment.  About
+// 3.6 VALU per evaluation against 6.
 // ---------------------------------------------------------------------------
 constexpr int kWtaRowMaxWaves = 16;
 
@@ -372,32 +375,35 @@ __device__ __forceinline__ void wta_row_segment(const uint64_t* ccS, const uint6
     const int cb = x0 + 4 * lane;
     uint64_t cci[4], w[4];
     uint32_t bL[4], bR[4], ba[4];
+    const int32_t m512 = -512;   // a register: the mad's other operand is the scalar ~d
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         cci[i] = (!RFIX || cb + i < C) ? ccS[cb + i] : 0;
         ba[i] = (RFIX && cb + i >= C) ? 128u : 0u;
         w[i] = cnS[cb + i + D - 1];
-        bL[i] = 0xFFFFFFFFu;
-        bR[i] = 0xFFFFFFFFu;
+        bL[i] = 0u;
+        bR[i] = 0u;
     }
     auto step = [&](int d) __attribute__((always_inline)) {
+        const uint32_t nd = ~(uint32_t)d;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const uint64_t x = cci[i] ^ w[i];
             const uint32_t pop = bcnt_add((uint32_t)(x >> 32), bcnt_add((uint32_t)x, ba[i]));
-            const uint32_t key = (pop << 9) | (uint32_t)d;
-            bL[i] = min(bL[i], (LFIX && d > cb + i) ? 0xFFFFFFFFu : key);
-            bR[i] = min(bR[i], key);
+            uint32_t nk;   // ~((pop << 9) | d) = ~d - 512 pop (pop <= 192)
+            asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(nk) : "v"(pop), "v"(m512), "s"(nd));
+            bL[i] = max(bL[i], (LFIX && d > cb + i) ? 0u : nk);
+            bR[i] = max(bR[i], nk);
         }
     };
     auto shift = [&](int d) __attribute__((always_inline)) {
         // to step d + 1: R accumulators move up one pixel (one lane per 4)
         const uint32_t out = bR[3];
-        const uint32_t in = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)out, 0x138, 0xF, 0xF, false);
+        const uint32_t in = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)out, 0x138, 0xF, 0xF, true);
         const int pl = x0 + 255 - d;          // pixel of lane 63's leaving accumulator
         // interior segments: x0 + 256 <= C and x0 >= D - 1, so 0 <= pl < C; at
         // the row's start pl < 0 once d > x0 + 255 (D > 256)
-        if (lane == 63 && (!LFIX || pl >= 0) && (!RFIX || pl < C)) atomicMin(&RP[pl], out);
+        if (lane == 63 && (!LFIX || pl >= 0) && (!RFIX || pl < C)) atomicMax(&RP[pl], out);
         bR[3] = bR[2]; bR[2] = bR[1]; bR[1] = bR[0]; bR[0] = in;
         w[3] = w[2]; w[2] = w[1]; w[1] = w[0];
         w[0] = cnS[cb + D - 2 - d];           // CN[cb - (d + 1)]
@@ -418,9 +424,9 @@ __device__ __forceinline__ void wta_row_segment(const uint64_t* ccS, const uint6
     step(d + 3);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        if (!RFIX || cb + i < C) G(outL)[cb + i] = (uint16_t)(bL[i] & 511u);
+        if (!RFIX || cb + i < C) G(outL)[cb + i] = (uint16_t)(~bL[i] & 511u);
         const int pp = cb + i - (D - 1);      // pixel of R[i] after the last step
-        if (pp >= 0 && pp < C) atomicMin(&RP[pp], bR[i]);
+        if (pp >= 0 && pp < C) atomicMax(&RP[pp], bR[i]);
     }
 }
 
@@ -450,7 +456,7 @@ __global__ __launch_bounds__(64 * kWtaRowMaxWaves) void k_wta_row(const RigDesc*
     }
     for (int i = threadIdx.x; i < C; i += blockDim.x) {
         ccS[i] = G(cc)[i];
-        RP[i] = 0xFFFFFFFFu;
+        RP[i] = 0u;   // complemented keys under max
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
@@ -468,7 +474,7 @@ __global__ __launch_bounds__(64 * kWtaRowMaxWaves) void k_wta_row(const RigDesc*
     }
     __syncthreads();
     uint16_t* const outR = g.wta[2 * F + 1] + (size_t)r * C;
-    for (int p = threadIdx.x; p < C; p += blockDim.x) G(outR)[p] = (uint16_t)(RP[p] & 511u);
+    for (int p = threadIdx.x; p < C; p += blockDim.x) G(outR)[p] = (uint16_t)(~RP[p] & 511u);
 }
 
 // The WTA maps of the deep padding bands' rows from each band's first row:
@@ -1272,15 +1278,15 @@ __device__ __forceinline__ uint4 pack16(const uint32_t (&p)[8])
     return make_uint4(pack4(p[0], p[1]), pack4(p[2], p[3]), pack4(p[4], p[5]), pack4(p[6], p[7]));
 }
 
-// DPP within a 16-lane row: lane i gets lane i-1 (shr) / i+1 (shl); `fill`
-// where the source is outside the row
-__device__ __forceinline__ uint32_t row_shr1(uint32_t v, uint32_t fill)
+// DPP within a 16-lane row: lane i gets lane i-1 (shr) / i+1 (shl); 0 where
+// the source is outside the row (one DPP move, no fill value to materialise)
+__device__ __forceinline__ uint32_t row_shr1(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x111, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);
 }
-__device__ __forceinline__ uint32_t row_shl1(uint32_t v, uint32_t fill)
+__device__ __forceinline__ uint32_t row_shl1(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x101, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x101, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_min(uint32_t v)
@@ -1353,8 +1359,11 @@ template <int CH, int KL, bool ACC, class MF, bool DEFER>
 __device__ __forceinline__ void lane_step_f(LaneState<CH>& st, const ChainGeo& cg, MF&& mpair,
                                             uint32_t (&tp)[CH][8], bool trunc)
 {
-    const uint32_t nbl = row_shr1(st.L[8 * CH - 1], kGuard2);   // left lane's last pair (old)
-    const uint32_t nbr = row_shl1(st.L[0], kGuard2);            // right lane's first pair (old)
+    // left lane's last pair and right lane's first pair (old); KL divides the
+    // 16-lane DPP row, so the lanes at a row's ends are group ends, which take
+    // the guard instead
+    const uint32_t nbl = row_shr1(st.L[8 * CH - 1]);
+    const uint32_t nbr = row_shl1(st.L[0]);
     uint32_t prv = (cg.lid == 0) ? kGuard2 : nbl;
     const uint32_t rfill = (cg.lid == KL - 1) ? kGuard2 : nbr;
     const uint32_t mpp = dup16(st.mp);
