@@ -1565,7 +1565,10 @@ __global__ __launch_bounds__(64) void k_fam(const RigDesc* __restrict__ rigs, in
                 static_for<0, CH>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
                     const bool ok = t < nsteps && valid && cg.lid * CH + q < cg.nq;
-                    b.m[q] = ok ? LD16(src + 16 * q) : make_uint4(0, 0, 0, 0);
+                    // not loaded: a padding chunk (its pairs take the guard whatever
+                    // bytes they hold), a chain past the rig (never stored) or a
+                    // step past the row (never processed): no zero fill
+                    if (ok) b.m[q] = LD16(src + 16 * q);
                 });
                 return;
             }
