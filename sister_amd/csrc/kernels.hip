@@ -164,6 +164,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int tilesC = (C + kCT - 1) / kCT;
     const int r0 = (t / tilesC) * kCT, c0 = (t % tilesC) * kCT;
     const uint8_t* v = g.view[(unit & 1) ? 1 + F : 0];
+    // rows of the deep padding bands other than their first rows are never
+    // stored (readers take the first row, band_row): a tile of such rows only
+    // has nothing to do
+    if (r0 + kCT - 1 < R && band_row(r0, R, g.D) != r0 && band_row(r0 + kCT - 1, R, g.D) != r0 + kCT - 1 &&
+        band_row(r0, R, g.D) == band_row(r0 + kCT - 1, R, g.D))
+        return;
 
     // frames 90/270 are transposes: step along frame rows there, so that
     // consecutive lanes read along an image row
@@ -224,6 +230,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         }
     };
     auto store_run = [&](int r, int cb, const uint64_t (&code)[4]) __attribute__((always_inline)) {
+        if (band_row(r, R, g.D) != r) return;                // a band's copy row: not stored
         uint64_t* out = g.code[unit] + (size_t)r * C + cb;   // 32-byte aligned: cb % 4 == 0
         ST16(out, make_uint4((uint32_t)code[0], (uint32_t)(code[0] >> 32), (uint32_t)code[1], (uint32_t)(code[1] >> 32)));
         ST16(out + 2, make_uint4((uint32_t)code[2], (uint32_t)(code[2] >> 32), (uint32_t)code[3], (uint32_t)(code[3] >> 32)));
@@ -275,15 +282,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // Lowest index wins ties: keys are (cost << 9) | d, reduced with min.
 // One block = one frame row segment of 256 pixels; codes staged in LDS.
 // ---------------------------------------------------------------------------
-// Frame rows of the deep padding bands, [3, D-4] and [R-D+3, R-4]: with
-// replicate padding every census code of such a row equals that of the band's
-// first row, in the centre and the neighbour frame alike (the window sees one
-// image row; see k_census), so their WTA maps equal the first row's.  The WTA
-// kernels compute the first row of each band and k_wta_fill copies it.
-__device__ __forceinline__ bool wta_band_copy(int r, int R, int D)
-{
-    return (r > 3 && r <= D - 4) || (r > R - D + 3 && r <= R - 4);
-}
+// Rows of the deep padding bands other than a band's first row (band_row):
+// the WTA kernels compute the first row of each band and k_wta_fill copies it.
+__device__ __forceinline__ bool wta_band_copy(int r, int R, int D) { return band_row(r, R, D) != r; }
 
 __global__ __launch_bounds__(256) void k_wta(const RigDesc* __restrict__ rigs, int nrig,
                                              const int* __restrict__ bs)
@@ -929,7 +930,7 @@ __global__ __launch_bounds__(256) void k_msum_run(const RigDesc* __restrict__ ri
     // frame row holding the line and the frame column of run pixel i:
     //  SIDE 0: F0 row y col x;  F2 row y col W'-1-x
     //  SIDE 1: F1 row W'-1-x col H'-1-y;  F3 row W'-1-x col y
-    const int frow = SIDE == 0 ? line : Wp - 1 - line;
+    const int frow = band_row(SIDE == 0 ? line : Wp - 1 - line, SIDE == 0 ? Hp : Wp, D);   // stored row
     const int fcols = SIDE == 0 ? Wp : Hp;
     const uint64_t* NA = g.code[2 * FA + 1] + (size_t)frow * fcols;
     const uint64_t* NB = g.code[2 * FB + 1] + (size_t)frow * fcols;
@@ -1071,7 +1072,7 @@ __global__ __launch_bounds__(1024) void k_msum_tile(const RigDesc* __restrict__ 
             const int l = wave;
             const bool rowf = (F & 1) == 0;
             const bool line_ok = rowf ? q.y0 + l < Hp : q.x0 + l < Wp;
-            const int frow = rowf ? q.y0 + l : Wp - 1 - (q.x0 + l);
+            const int frow = band_row(rowf ? q.y0 + l : Wp - 1 - (q.x0 + l), rowf ? Hp : Wp, D);   // stored row
             const int fcols = rowf ? Wp : Hp;
             const int lo = F == 0 ? q.x0 - (D - 1)
                          : F == 2 ? Wp - q.x0 - kMT - (D - 1)
@@ -1090,7 +1091,7 @@ __global__ __launch_bounds__(1024) void k_msum_tile(const RigDesc* __restrict__ 
 #pragma unroll
             for (int F = 0; F < 4; F++) {
                 const bool rowf = (F & 1) == 0;
-                const int frow = rowf ? y : Wp - 1 - x;
+                const int frow = band_row(rowf ? y : Wp - 1 - x, rowf ? Hp : Wp, D);   // stored row
                 const int fcol = F == 0 ? x : F == 2 ? Wp - 1 - x : F == 1 ? Hp - 1 - y : y;
                 const int fcols = rowf ? Wp : Hp;
                 // a set mask implies frame column >= D; the check keeps indices in range

@@ -53,4 +53,16 @@ struct RigDesc {
 __host__ __device__ inline int frame_rows(const RigDesc& r, int F) { return (F & 1) ? r.Wp : r.Hp; }
 __host__ __device__ inline int frame_cols(const RigDesc& r, int F) { return (F & 1) ? r.Hp : r.Wp; }
 
+// Deep padding bands: with replicate padding every census code of a frame
+// row in [3, D-4] or [R-D+3, R-4] (R frame rows) equals that of the band's
+// first row, in the centre and the neighbour frame alike (the 9x7 window sees
+// one image row), and so do the row's WTA maps.  band_row gives the row whose
+// codes and maps row r holds: the band's first row inside a band, r elsewhere.
+// The census stores only first rows, the WTA computes only first rows, and
+// readers of either go through band_row.
+__host__ __device__ inline int band_row(int r, int R, int D)
+{
+    return (r > 3 && r <= D - 4) ? 3 : (r > R - D + 3 && r <= R - 4) ? R - D + 3 : r;
+}
+
 }  // namespace sister
